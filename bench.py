@@ -1,0 +1,203 @@
+"""Benchmark of the MI355X subset-DP hafnian path.
+
+Workload (one "step"): the forward walk of a batch of B = 1024 real Float64 symmetric N = 28
+matrices (entries N(0, 1/N), seeded) sharing one plan, with P and the results resident in HBM
+when the timed region starts.  Reported metric: DP transitions per second (one transition = one
+(matrix, transition) multiply-add, 6 052 062 per N = 28 matrix), with hafnians/s, the plain
+one-matrix-per-call path at the same N (Float64, and the ComplexF64 single-hafnian case), the
+HBM roofline of the level kernel and the CPU restatement of the reference timed on the host.
+
+    python bench.py [--steps K] [--warmup W] [--batch B] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "theeggman.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0
+N_DEFAULT = 28
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def alg_bytes_fwd(K, B, tbytes, ibytes, ns, nt):
+    """(ntrans + nstates + 1) * sizeof(I) + B * (ntrans + nstates + npairs) * sizeof(T): the
+    plan streamed once per batch; per matrix one child read per transition, one write per
+    state and one read per pair."""
+    npairs = K * (K - 1) // 2
+    return (nt + ns + 1) * ibytes + B * (nt + ns + npairs) * tbytes
+
+
+def time_device(hip, stream, fn, steps, warmup):
+    """Run fn warmup times, then time exactly `steps` calls between synchronisations, with HIP
+    events recorded on `stream`, the stream the kernels are launched on."""
+    for _ in range(warmup):
+        fn()
+    hip.device_synchronize()
+    ev0, ev1 = hip.Event(), hip.Event()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(steps):
+        fn()
+    ev1.record(stream)
+    ev1.synchronize()
+    hip.device_synchronize()
+    wall = time.perf_counter() - t0
+    return ev0.elapsed_ms(ev1) / steps, wall * 1e3 / steps
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the level kernel from the committed rocprofv3 PMC summary, if
+    present (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_n28_b1024.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(K, budget_s=12.0):
+    """The oracle's C restatement of the reference CPU walk (_batch_cpu! over _haf_dp,
+    src/hafnian.jl:1226-1242, src/dp.jl:408-442) on this host's cores: min over reps after one
+    warmup, plan build excluded."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle
+    from theeggman import synth
+    nthreads = min(os.cpu_count() or 1, oracle.max_threads())
+    Bs = max(nthreads * 8, 64)
+    As = np.stack([synth.sym_normal(K, 3, b, 1 / np.sqrt(K)) for b in range(Bs)])
+    P = synth.pair_batch(As)
+    oracle.build_plan(K)
+    oracle.haf_dp(P, K, oracle.F64, nthreads)            # warmup
+    best, spent, reps = float("inf"), 0.0, 0
+    while spent < budget_s and reps < 50:
+        t0 = time.perf_counter()
+        oracle.haf_dp(P, K, oracle.F64, nthreads)
+        dt = time.perf_counter() - t0
+        best, spent, reps = min(best, dt), spent + dt, reps + 1
+    _, nt = oracle.dp_counts(K)
+    return {"value": Bs * nt / best, "unit": "DP transitions/s", "cores": nthreads,
+            "kind": "port", "hafnians_per_s": Bs / best,
+            "sample": f"{Bs} real Float64 N={K} matrices per rep, _batch_cpu! policy over "
+                      f"{nthreads} threads, min of {reps} reps ({spent:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=1024)
+    ap.add_argument("--N", type=int, default=N_DEFAULT)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain-path lines")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): the batched path "
+              "shards by instance with no communication, so its whole-job rate is N times the "
+              "one-GPU value", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import theeggman as te
+    from theeggman import hip, synth
+
+    K, B = args.N, args.batch
+    be = te.HIPBackend()
+    plan = te._dp_plan(K)
+    ns, nt = te._dp_counts(K)
+    ib = plan.index_bytes
+    stream = hip.Stream()
+
+    log(f"building inputs N={K} B={B}")
+    As = np.stack([synth.sym_normal(K, 3, b, 1 / np.sqrt(K)) for b in range(B)])
+    P = synth.pair_batch(As)                                    # [pair][batch]
+    Pd = hip.DeviceBuffer.from_array(P)
+    out = hip.DeviceBuffer(B * 8)
+
+    def step():
+        te._haf_dp_device(be, plan, np.float64, Pd.addr, B, B, out.addr, stream.handle)
+
+    log("batched forward")
+    ms, wall_ms = time_device(hip, stream, step, args.steps, args.warmup)
+    # spot-check the timed result against the oracle (bit-exact)
+    res = out.download(np.float64, B)
+    check = None
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        idx = [0, B // 2, B - 1]
+        want = oracle.haf_dp(P[:, idx].copy(), K, oracle.F64, 8)
+        check = bool(res[idx].tobytes() == want.tobytes())
+    except Exception as e:  # the oracle is a checker only; its absence does not stop the bench
+        log(f"oracle spot check skipped: {e}")
+    if check is False:
+        raise SystemExit("device result differs from the oracle: refusing to report")
+
+    value = B * nt / (ms / 1e3)
+    nbytes = alg_bytes_fwd(K, B, 8, ib, ns, nt)
+    launches = K // 2
+    achieved = nbytes / (ms / 1e3) / 1e9
+    result = {
+        "metric": "DP transitions/s (batched N=28 FP64 forward)",
+        "value": value,
+        "unit": "transitions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64 normals, N(0,1/N), symmetric, zero diagonal)",
+        "config": {"workload": f"batched forward, {B} real Float64 N={K} matrices sharing one "
+                               f"Int32 plan, inputs resident in HBM", "N": K, "batch": B,
+                   "plan_bytes": (nt + ns + 1) * ib},
+        "hafnians_per_s": B / (ms / 1e3),
+        "wall_ms_per_step": wall_ms,
+        "oracle_spot_check": check,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
+                     "kernel": "k_level_fwd", "launches_per_step": launches,
+                     "alg_bytes_per_launch": nbytes / launches,
+                     "avg_launch_ms": ms / launches},
+    }
+
+    if not args.no_plain:
+        for name, T, gen in (("plain_f64", np.float64, lambda: synth.sym_normal(K, 3, 0, 1 / np.sqrt(K))),
+                             ("plain_c64", np.complex128, lambda: synth.sym_complex_normal(K, 2, 0))):
+            A = gen()
+            P1 = synth.pair_batch(A[None]).astype(T)
+            P1d = hip.DeviceBuffer.from_array(P1)
+            o1 = hip.DeviceBuffer(16)
+
+            def one():
+                te._haf_dp_device(be, plan, T, P1d.addr, 1, 1, o1.addr, stream.handle)
+            log(f"{name}")
+            steps1 = max(args.steps * 5, 50)
+            ms1, wall1 = time_device(hip, stream, one, steps1, args.warmup)
+            result[name] = {"ms_per_hafnian": ms1, "wall_ms_per_hafnian": wall1,
+                            "hafnians_per_s": 1e3 / ms1, "transitions_per_s": nt / (ms1 / 1e3)}
+        result["batched_vs_plain_f64_hafnians"] = result["hafnians_per_s"] / result["plain_f64"]["hafnians_per_s"]
+
+    if not args.no_cpu:
+        log("cpu baseline")
+        result["cpu_baseline"] = cpu_baseline(K)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,149 @@
+/*
+ * eggman_hip.h -- C ABI of the MI355X (gfx950) engine for TheEggman.jl's subset-DP hafnian path.
+ *
+ * The library replaces the device hook `_haf_dp_backend(backend, P, plan; max_batch)`
+ * (src/dp.jl:514-531, implemented for KernelAbstractions at
+ * ext/TheEggmanKernelAbstractionsExt.jl:90-146) together with the device caches around it
+ * (src/dp.jl:533-587).  Everything upstream -- validation, strategy choice, the gather into P,
+ * and the plan -- stays with the caller.
+ *
+ * Conventions
+ *   - Every function returns EGGMAN_OK (0) or a negative status; eggman_last_error() then
+ *     describes the failure (thread-local).
+ *   - Plan arrays keep the reference's 1-based values and packing: a transition is
+ *     (child << bits) | pair with bits = 9 for 4-byte and 12 for 8-byte words
+ *     (src/dp.jl:150-180).
+ *   - P is laid out [pair][batch] with the batch index fastest: element (b, q) of Julia's
+ *     column-major Matrix{T}(B, npairs) from _gather_pairs_batch (src/dp.jl:464-512) is at
+ *     q*B + b (0-based).  Complex values are interleaved (re, im).
+ *   - Host-pointer entry points are synchronous on return; device-pointer entry points are
+ *     asynchronous on the given HIP stream (NULL = the library's own stream for that device).
+ *   - All calls are thread-safe: one mutex serialises use of the library's device buffers, as
+ *     the reference's _DEVICE_PLAN_LOCK does (ext/TheEggmanKernelAbstractionsExt.jl:117-119).
+ */
+#ifndef EGGMAN_HIP_H
+#define EGGMAN_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define EGGMAN_OK 0
+#define EGGMAN_EINVAL (-1)       /* bad argument (maps to ArgumentError / DimensionMismatch) */
+#define EGGMAN_ENOMEM (-2)       /* host or device allocation failed, or would not fit */
+#define EGGMAN_EPLAN (-3)        /* plan cannot be represented, or is inconsistent */
+#define EGGMAN_EHIP (-4)         /* a HIP runtime call failed */
+#define EGGMAN_EUNSUPPORTED (-5) /* valid request this build does not handle */
+
+/* element types (the T of Matrix{T}) */
+#define EGGMAN_F32 1  /* Float32 */
+#define EGGMAN_F64 2  /* Float64 */
+#define EGGMAN_C32 3  /* ComplexF32, interleaved */
+#define EGGMAN_C64 4  /* ComplexF64, interleaved */
+#define EGGMAN_I128 5 /* exact: P holds int64 entries; results are 128-bit two's-complement
+                         integers (lo, hi uint64 limbs), exact modulo 2^128 */
+
+/* Version string of the library. */
+const char *eggman_version(void);
+
+/* Message for the last failing call on this thread ("" if none). */
+const char *eggman_last_error(void);
+
+/* Number of visible HIP devices (0 when none; never an error). */
+int eggman_device_count(void);
+
+/* ---------------------------------------------------------------- plan (host side) */
+
+/* Closed-form plan size for degree K: replaces _dp_counts (src/dp.jl:192-208). */
+int eggman_dp_counts(int K, int64_t *nstates, int64_t *ntrans);
+
+/*
+ * Build the degree-K plan on the host into caller arrays of int32 (index_bytes 4) or int64 (8):
+ * starts[nstates+1], trans[ntrans], levels[K/2+2].  Same values, order and packing as
+ * _build_dp_plan (src/dp.jl:223-304), including its refusals (K > 64, packing overflow) --
+ * computed by combinatorial ranking instead of a set and a dictionary, on `nthreads` host
+ * threads (0 = all).
+ */
+int eggman_plan_build(int K, int index_bytes, void *starts, void *trans, void *levels,
+                      int nthreads);
+
+/* ---------------------------------------------------------------- plan (device side) */
+
+/*
+ * Upload a caller-built plan (borrowed; read only during the call) to the current device and
+ * return a handle.  Replaces _device_plan (ext/TheEggmanKernelAbstractionsExt.jl:31-42): the
+ * device copy is cached per (device, K, index_bytes), so uploading the same degree again returns
+ * the cached handle without copying.  Rejects (EGGMAN_EPLAN) arrays that are not a valid plan
+ * layout (sizes, levels, uniform out-degree per level).
+ */
+int eggman_plan_upload(int K, int index_bytes, const void *starts, int64_t nstarts,
+                       const void *trans, int64_t ntrans, const void *levels, int64_t nlevels,
+                       uint64_t *handle);
+
+/*
+ * Generate the degree-K plan directly in device memory (no host build, no upload) and return a
+ * handle; bit-identical to the uploaded plan.  Cached like eggman_plan_upload.  New: the
+ * reference has no device-side plan generation.
+ */
+int eggman_plan_generate(int K, int index_bytes, uint64_t *handle);
+
+/*
+ * Check caller-supplied plan arrays without touching a device: the sizes and level boundaries
+ * the closed form gives, one out-degree per level, and every transition pointing into the level
+ * below with a pair index in 1..npairs.  eggman_plan_upload runs the same check; a plan that
+ * fails it is never walked.  Mirrors the invariants of test/runtests.jl:241-267.
+ */
+int eggman_plan_validate(int K, int index_bytes, const void *starts, int64_t nstarts,
+                         const void *trans, int64_t ntrans, const void *levels, int64_t nlevels);
+
+/* Degree, index width, state and transition counts of a plan handle. */
+int eggman_plan_info(uint64_t handle, int *K, int *index_bytes, int64_t *nstates,
+                     int64_t *ntrans);
+
+/* Copy a device plan's transitions back to the host (ntrans words), for verification. */
+int eggman_plan_download(uint64_t handle, void *trans_host, int64_t ntrans);
+
+/* ---------------------------------------------------------------- forward walk */
+
+/*
+ * Hafnians of the B instances in P_host ([pair][batch], npairs = K(K-1)/2 rows) into out_host
+ * (B elements; 16 bytes each for EGGMAN_I128).  Replaces
+ * _haf_dp_backend(backend, P, plan; max_batch) (ext/TheEggmanKernelAbstractionsExt.jl:90-146):
+ * batches are processed in chunks of max_batch instances (<= 0: the whole batch); the result is
+ * bit-identical for every chunking, and every state's sum is accumulated in plan order without
+ * fused multiply-add, exactly as _dp_states! (src/dp.jl:344-363) does.  B == 0 is a no-op.
+ */
+int eggman_dp_forward(uint64_t plan, int dtype, const void *P_host, int64_t B, int64_t npairs,
+                      int64_t max_batch, void *out_host);
+
+/*
+ * Device-resident form of eggman_dp_forward: P_dev is [pair][ldp] in device memory (instance b
+ * of pair q at q*ldp + b, ldp >= B), out_dev receives B results.  Asynchronous on `stream`
+ * (a hipStream_t; NULL = the library's stream).  No host transfer happens.  New: the
+ * reference only exposes the host form.
+ */
+int eggman_dp_forward_device(uint64_t plan, int dtype, const void *P_dev, int64_t ldp,
+                             int64_t B, void *out_dev, void *stream);
+
+/* ---------------------------------------------------------------- caches */
+
+/* Device bytes held by cached plans and work buffers: the library's share of gpu_cache_bytes
+ * (src/dp.jl:552-560). */
+int64_t eggman_cache_bytes(void);
+
+/* Release every cached plan and work buffer: empty_gpu_cache! (src/dp.jl:567-573).  Existing
+ * handles become invalid. */
+int eggman_cache_clear(void);
+
+/* Device memory one batch needs besides the shared plan, B*(nstates+npairs)*sizeof(T):
+ * dp_batch_bytes (src/dp.jl:584-587). */
+int64_t eggman_batch_bytes(int K, int dtype, int64_t B);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EGGMAN_HIP_H */

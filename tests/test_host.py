@@ -1,0 +1,182 @@
+"""Host-side checks of the product that need no GPU: the C ABI exports, the library's host plan
+builder and validator against the oracle, and the router's validation and refusals (which
+raise before any device call)."""
+import ctypes
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as o
+import theeggman as te
+from theeggman import _lib, synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = json.load(open(os.path.join(ROOT, "tests", "golden", "golden.json")))
+
+
+def header_symbols():
+    src = open(os.path.join(ROOT, "include", "eggman_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(eggman_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    syms = header_symbols()
+    assert len(syms) >= 10
+    for name in syms:
+        assert hasattr(L, name), name
+        assert name in _lib.SIGNATURES, f"{name} has no ctypes signature"
+    assert set(_lib.SIGNATURES) == set(syms)
+
+
+def test_version_and_device_count():
+    L = _lib.lib()
+    assert b"gfx950" in L.eggman_version()
+    assert L.eggman_device_count() >= 0
+
+
+@pytest.mark.parametrize("K", list(range(2, 65, 2)))
+def test_counts_equal_oracle(K):
+    assert te._dp_counts(K) == o.py_dp_counts(K)
+
+
+@pytest.mark.parametrize("K", list(range(2, 29, 2)))
+@pytest.mark.parametrize("I", [np.int32, np.int64])
+def test_host_builder_equals_oracle(K, I):
+    p = te._build_dp_plan(K, I)
+    q = o.build_plan(K, np.dtype(I).itemsize)
+    assert np.array_equal(p.starts, q.starts)
+    assert np.array_equal(p.trans, q.trans)
+    assert np.array_equal(p.levels, q.levels)
+    if K <= 24:
+        assert q.sha256() == GOLD["plan_sha256"][f"{K}_{np.dtype(I).itemsize}"]
+
+
+def test_host_builder_k32():
+    p = te._build_dp_plan(32, np.int32)
+    q = o.build_plan(32, 4)
+    assert np.array_equal(p.trans, q.trans) and np.array_equal(p.starts, q.starts)
+
+
+def test_plan_cache_and_layouts():
+    # test/runtests.jl:263-264
+    assert te._dp_plan(8) is te._dp_plan(8)
+    assert te._dp_plan(8, np.int32) is not te._dp_plan(8, np.int64)
+    assert te._dp_index_type(32) == np.int32 and te._dp_index_type(34) == np.int64
+
+
+def test_build_refusals():
+    # test/runtests.jl:293-297
+    with pytest.raises(te.ArgumentError):
+        te._build_dp_plan(66, np.int64)
+    with pytest.raises(te.ArgumentError):
+        te._build_dp_plan(50, np.int64)        # 3.3 TB: more than this machine has
+    with pytest.raises(te.ArgumentError):
+        te._build_dp_plan(34, np.int32)        # will not pack
+
+
+def _plan_args(p):
+    return (p.K, p.index_bytes, p.starts.ctypes.data, p.starts.size, p.trans.ctypes.data,
+            p.trans.size, p.levels.ctypes.data, p.levels.size)
+
+
+def test_validator_accepts_plans_and_rejects_corruption():
+    L = _lib.lib()
+    for K in (4, 12, 20):
+        for I in (np.int32, np.int64):
+            p = te._build_dp_plan(K, I)
+            assert L.eggman_plan_validate(*_plan_args(p)) == 0
+    p = te._build_dp_plan(12, np.int32)
+    bad = te.HafnianDPPlan(p.K, p.nstates, p.starts.copy(), p.trans.copy(), p.levels.copy())
+    bad.trans[100] = (bad.trans[100] & 511) | ((p.nstates + 5) << 9)   # child out of range
+    assert L.eggman_plan_validate(*_plan_args(bad)) == _lib.EPLAN
+    bad = te.HafnianDPPlan(p.K, p.nstates, p.starts.copy(), p.trans.copy(), p.levels.copy())
+    bad.trans[7] = (bad.trans[7] >> 9) << 9                            # pair 0
+    assert L.eggman_plan_validate(*_plan_args(bad)) == _lib.EPLAN
+    bad = te.HafnianDPPlan(p.K, p.nstates, p.starts.copy(), p.trans.copy(), p.levels.copy())
+    bad.starts[50] += 1                                                # ragged out-degree
+    assert L.eggman_plan_validate(*_plan_args(bad)) == _lib.EPLAN
+    bad = te.HafnianDPPlan(p.K, p.nstates, p.starts.copy(), p.trans.copy(), p.levels.copy())
+    bad.levels[3] += 1
+    assert L.eggman_plan_validate(*_plan_args(bad)) == _lib.EPLAN
+    assert "level" in _lib.last_error()
+
+
+def test_dp_batch_bytes():
+    # test/gpu.jl:149-154
+    ns, _ = te._dp_counts(20)
+    assert te.dp_batch_bytes(20, np.complex128, 1) == (ns + 190) * 16
+    assert te.dp_batch_bytes(20, np.complex64, 8) == 8 * (ns + 190) * 8
+    assert _lib.lib().eggman_batch_bytes(20, _lib.C64, 1) == (ns + 190) * 16
+
+
+def test_gather_matches_oracle():
+    A = synth.sym_uniform(9, 7, 0)
+    P = te._gather_pairs_batch(np.float64, A, None, 9)
+    assert P.shape == (1, 36)
+    assert np.array_equal(P[0], o.gather_pairs(A))
+    idx = [1, 1, 2, 3, 3, 3]
+    P = te._gather_pairs_batch(np.float64, A, [idx], 6)
+    assert np.array_equal(P[0], o.gather_pairs(A, idx))
+    As = np.stack([synth.sym_uniform(6, 7, b) for b in range(3)])
+    P = te._gather_pairs_batch(np.float64, As, None, 6)
+    assert P.flags.f_contiguous and np.array_equal(P.T, synth.pair_batch(As))
+
+
+def test_router_validation_before_any_device_call():
+    # src/hafnian.jl:1058-1066, 1111-1147, 1263-1301, 1317-1359; test/gpu.jl:110-119
+    A = synth.sym_uniform(10, 1, 0)
+    assert te.hafnian(np.zeros((0, 0))) == 1.0
+    assert te.hafnian(synth.sym_uniform(5, 1, 1)) == 0.0
+    with pytest.raises(te.DimensionMismatch):
+        te.hafnian(np.zeros((3, 4)))
+    with pytest.raises(te.ArgumentError):
+        te.hafnian(A, method="fast")
+    with pytest.raises(te.ArgumentError):
+        te.hafnian(np.ones((66, 66)), method="dp")
+    with pytest.raises(te.ArgumentError):
+        te.hafnian(np.ones((14, 14)), method="unrolled")
+    with pytest.raises(NotImplementedError):
+        te.hafnian(A, method="sieve")
+    with pytest.raises(NotImplementedError):
+        te.hafnian(np.ones((34, 34)))           # :auto above DP_MAX is the CPU sieve
+    bad = A.copy()
+    bad[0, 1] += 1
+    with pytest.raises(te.ArgumentError, match="not symmetric at"):
+        te.hafnian(bad)
+    with pytest.raises(te.ArgumentError):
+        te.hafnian_repeated(A, [np.ones(10, int), np.r_[1, 1, 1, 1, 1, 1, 1, 1, 1, 0]])
+    with pytest.raises(te.ArgumentError):
+        te.hafnian_repeated(A, [np.ones(10, int), np.r_[-1, np.ones(9, int)]])
+    with pytest.raises(te.DimensionMismatch):
+        te.hafnian_repeated(A, [np.ones(10, int), np.ones(9, int)])
+    with pytest.raises(te.DimensionMismatch):
+        te.hafnian([A, synth.sym_uniform(12, 1, 2)])
+    assert te.hafnian_repeated(A, np.zeros(10, int)) == 1.0
+    assert te.hafnian_repeated(A, np.r_[1, np.zeros(9, int)]) == 0.0
+    assert te.hafnian([]).shape == (0,)
+    assert te.hafnian_exact(np.ones((3, 3), int)) == 0
+    assert te.hafnian_exact(np.zeros((0, 0), int)) == 1
+    with pytest.raises(te.ArgumentError):
+        te.hafnian_exact(np.ones((4, 4)))       # not an integer matrix
+
+
+def test_reduction():
+    A = np.array([[1.0, 2.0], [2.0, 3.0]])
+    R = te.reduction(A, [2, 2])
+    assert R.shape == (4, 4) and R[0, 1] == 1.0 and R[0, 2] == 2.0 and R[2, 3] == 3.0
+
+
+def test_synthetic_generator_is_stable():
+    # fixed outputs of the documented generator (SplitMix64, 2^-53 uniform, 2^-30 normals)
+    x = synth.splitmix64(0, 2)
+    assert int(x[0]) == 0xE220A8397B1DCDAF and int(x[1]) == 0x6E789E6AA1B965F4
+    u = synth.uniform(synth.stream(3, 5), 1000)
+    assert (u >= -1).all() and (u < 1).all()
+    g = synth.normal(synth.stream(2, 0), 20000)
+    assert abs(g.mean()) < 0.03 and abs(g.std() - 1) < 0.03
+    assert (np.round(g * 2**30) == g * 2**30).all()

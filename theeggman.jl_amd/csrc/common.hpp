@@ -1,0 +1,119 @@
+// Shared definitions of the eggman HIP library: status handling and element-type traits.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdarg.h>
+#include <string>
+
+#include "../../include/eggman_hip.h"
+
+namespace eggman {
+
+// Sets the thread-local error message and returns `code`.
+int fail(int code, const char *fmt, ...);
+
+#define EGG_HIP(call)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return ::eggman::fail(e_ == hipErrorOutOfMemory ? EGGMAN_ENOMEM : EGGMAN_EHIP, \
+                                  "%s failed: %s", #call, hipGetErrorString(e_));          \
+    } while (0)
+
+#define EGG_TRY(expr)            \
+    do {                         \
+        int rc_ = (expr);        \
+        if (rc_ != EGGMAN_OK)    \
+            return rc_;          \
+    } while (0)
+
+// ------------------------------------------------------------------ element types on device
+//
+// Every muladd below is written as a separate multiply and add: the sources are compiled with
+// fp-contract off, so no fused multiply-add is formed and a state's sum matches the reference's
+// _dp_states! (src/dp.jl:344-363) bit for bit when accumulated in plan order.  Complex multiply
+// is (ac - bd, ad + bc), Julia's `*` for Complex.
+
+struct c64 { double re, im; };
+struct c32 { float re, im; };
+struct u128v { uint64_t lo, hi; };   // two's-complement 128-bit value, wrapping
+
+template <int DT> struct Elt;
+
+template <> struct Elt<EGGMAN_F64> {
+    using H = double;   // state value
+    using P = double;   // pair value
+    __device__ static inline H zero() { return 0.0; }
+    __device__ static inline H one() { return 1.0; }
+    __device__ static inline H muladd(H acc, P p, H h) { return acc + p * h; }
+};
+template <> struct Elt<EGGMAN_F32> {
+    using H = float;
+    using P = float;
+    __device__ static inline H zero() { return 0.0f; }
+    __device__ static inline H one() { return 1.0f; }
+    __device__ static inline H muladd(H acc, P p, H h) { return acc + p * h; }
+};
+template <> struct Elt<EGGMAN_C64> {
+    using H = double2;
+    using P = double2;
+    __device__ static inline H zero() { return make_double2(0.0, 0.0); }
+    __device__ static inline H one() { return make_double2(1.0, 0.0); }
+    __device__ static inline H muladd(H acc, P p, H h) {
+        const double re = p.x * h.x - p.y * h.y;
+        const double im = p.x * h.y + p.y * h.x;
+        return make_double2(acc.x + re, acc.y + im);
+    }
+};
+template <> struct Elt<EGGMAN_C32> {
+    using H = float2;
+    using P = float2;
+    __device__ static inline H zero() { return make_float2(0.0f, 0.0f); }
+    __device__ static inline H one() { return make_float2(1.0f, 0.0f); }
+    __device__ static inline H muladd(H acc, P p, H h) {
+        const float re = p.x * h.x - p.y * h.y;
+        const float im = p.x * h.y + p.y * h.x;
+        return make_float2(acc.x + re, acc.y + im);
+    }
+};
+// Exact integer walk: states are 128-bit integers modulo 2^128, pair values are int64.  The
+// DP only adds and multiplies, so the result is exact whenever the true hafnian fits in 127
+// bits, however large the intermediate values get.
+template <> struct Elt<EGGMAN_I128> {
+    using H = ulonglong2;   // (x = lo, y = hi)
+    using P = long long;
+    __device__ static inline H zero() { return make_ulonglong2(0ull, 0ull); }
+    __device__ static inline H one() { return make_ulonglong2(1ull, 0ull); }
+    __device__ static inline H muladd(H acc, P p, H h) {
+        const unsigned long long pl = (unsigned long long)p;
+        const unsigned long long ph = p < 0 ? ~0ull : 0ull;     // sign extension
+        const unsigned long long lo = pl * h.x;
+        const unsigned long long hi = __umul64hi(pl, h.x) + pl * h.y + ph * h.x;
+        const unsigned long long rlo = acc.x + lo;
+        const unsigned long long rhi = acc.y + hi + (rlo < lo ? 1ull : 0ull);
+        return make_ulonglong2(rlo, rhi);
+    }
+};
+
+inline int elt_hbytes(int dtype) {
+    switch (dtype) {
+    case EGGMAN_F32: return 4;
+    case EGGMAN_F64: return 8;
+    case EGGMAN_C32: return 8;
+    case EGGMAN_C64: return 16;
+    case EGGMAN_I128: return 16;
+    }
+    return 0;
+}
+inline int elt_pbytes(int dtype) {
+    switch (dtype) {
+    case EGGMAN_F32: return 4;
+    case EGGMAN_F64: return 8;
+    case EGGMAN_C32: return 8;
+    case EGGMAN_C64: return 16;
+    case EGGMAN_I128: return 8;
+    }
+    return 0;
+}
+
+}  // namespace eggman

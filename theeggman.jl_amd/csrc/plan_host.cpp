@@ -1,0 +1,195 @@
+// Host side of the plan: closed-form counts and layout, the host builder, and plan validation.
+//
+// The values written here are those of the reference's _build_dp_plan (src/dp.jl:248-304):
+// states numbered by (subset size, mask value), transitions per state in ascending partner
+// order, packed (child << bits) | pair with 1-based child and pair, starts/levels 1-based.
+// The construction is different: each level's reachable subsets are all m-subsets of one
+// ground set (see plan_rank.hpp), so states are produced in order by stepping through colex
+// order and children are ranked arithmetically -- no set, no sort, no dictionary, and every
+// level can be split across threads.
+#include <algorithm>
+#include <atomic>
+#include <thread>
+#include <vector>
+
+#include "plan_host.hpp"
+
+namespace eggman {
+
+static constexpr Binom kHostBinom = make_binom();
+const Binom &host_binom() { return kHostBinom; }
+
+int pair_bits_for(int index_bytes) { return index_bytes == 4 ? 9 : 12; }   // src/dp.jl:170-171
+
+int plan_layout(int K, int index_bytes, PlanLayout *lay) {
+    if (K < 2 || K > kMaxK || (K & 1))
+        return fail(EGGMAN_EINVAL, "degree K=%d must be even and in [2, %d]", K, kMaxK);
+    if (index_bytes != 4 && index_bytes != 8)
+        return fail(EGGMAN_EINVAL, "index_bytes must be 4 or 8, got %d", index_bytes);
+    const uint64_t(*C)[kMaxK + 1] = kHostBinom.c;
+    lay->K = K;
+    lay->index_bytes = index_bytes;
+    lay->bits = pair_bits_for(index_bytes);
+    lay->npairs = (int64_t)K * (K - 1) / 2;
+    lay->nlevels = K / 2 + 2;
+    lay->levels.assign(lay->nlevels, 0);
+    lay->tstart.assign(lay->nlevels, 0);
+    lay->size.assign(lay->nlevels, 0);
+    lay->lo.assign(lay->nlevels, 0);
+    lay->nst.assign(lay->nlevels, 0);
+    int64_t s = 1, t = 0;
+    for (int L = 1; L <= K / 2 + 1; L++) {          // level L holds subsets of size 2(L-1)
+        const int m = 2 * (L - 1);
+        const int lo = ground_lo(K, m);
+        const int n = K - lo + 1;
+        const int64_t cnt = (int64_t)C[n][m];
+        lay->levels[L - 1] = s;
+        lay->tstart[L - 1] = t;
+        lay->size[L - 1] = m;
+        lay->lo[L - 1] = lo;
+        lay->nst[L - 1] = cnt;
+        s += cnt;
+        t += cnt * (m > 0 ? m - 1 : 0);
+    }
+    lay->levels[lay->nlevels - 1] = s;
+    lay->tstart[lay->nlevels - 1] = t;
+    lay->nstates = s - 1;
+    lay->ntrans = t;
+    lay->max_level = 0;
+    for (int L = 1; L <= K / 2 + 1; L++) lay->max_level = std::max(lay->max_level, lay->nst[L - 1]);
+
+    // the refusals of _build_dp_plan (src/dp.jl:224-234)
+    const int64_t imax = index_bytes == 4 ? INT32_MAX : INT64_MAX;
+    if (lay->npairs > ((int64_t)1 << lay->bits) - 1)
+        return fail(EGGMAN_EPLAN, "degree %d needs more than %d bits per pair index in Int%d", K,
+                    lay->bits, index_bytes * 8);
+    if (lay->nstates > (imax >> lay->bits))
+        return fail(EGGMAN_EPLAN, "degree %d has too many states to pack into an Int%d transition",
+                    K, index_bytes * 8);
+    if (lay->ntrans > imax)
+        return fail(EGGMAN_EPLAN, "degree %d has too many transitions to index with Int%d", K,
+                    index_bytes * 8);
+    return EGGMAN_OK;
+}
+
+// Fill the transitions of level L (1-based) for ranks [r0, r1) into `trans` (whole-plan array)
+// and the matching `starts` entries.
+template <typename W>
+static void build_level_range(const PlanLayout &lay, int L, int64_t r0, int64_t r1, W *trans,
+                              W *starts) {
+    const uint64_t(*C)[kMaxK + 1] = kHostBinom.c;
+    const int m = lay.size[L - 1];
+    const int lo = lay.lo[L - 1];
+    const int n = lay.K - lo + 1;
+    const int k = m - 1;
+    const int64_t first = lay.levels[L - 1];
+    const int64_t child_first = lay.levels[L - 2];
+    int x[kMaxK];
+    colex_unrank(C, n, m, (uint64_t)r0, x);
+    for (int64_t r = r0; r < r1; r++) {
+        const int64_t t0 = lay.tstart[L - 1] + r * k;
+        state_transitions<W>(C, lay.K, m, lo, x, child_first, lay.bits, trans + t0);
+        starts[first + r] = (W)(t0 + k + 1);     // starts[s+1] in 1-based terms
+        if (r + 1 < r1) colex_next(m, x);
+    }
+}
+
+template <typename W>
+static int build_plan_t(const PlanLayout &lay, W *starts, W *trans, W *levels, int nthreads) {
+    for (int L = 0; L < lay.nlevels; L++) levels[L] = (W)lay.levels[L];
+    starts[0] = 1;
+    starts[1] = 1;                                   // the empty set has no transitions
+    if (nthreads <= 0) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
+    struct Job { int L; int64_t r0, r1; };
+    std::vector<Job> jobs;
+    const int64_t chunk = 1 << 15;
+    for (int L = 2; L <= lay.K / 2 + 1; L++)
+        for (int64_t r = 0; r < lay.nst[L - 1]; r += chunk)
+            jobs.push_back({L, r, std::min(lay.nst[L - 1], r + chunk)});
+    std::atomic<size_t> next(0);
+    auto worker = [&]() {
+        for (;;) {
+            const size_t j = next.fetch_add(1);
+            if (j >= jobs.size()) break;
+            build_level_range<W>(lay, jobs[j].L, jobs[j].r0, jobs[j].r1, trans, starts);
+        }
+    };
+    const int nt = (int)std::min<size_t>((size_t)nthreads, std::max<size_t>(1, jobs.size()));
+    if (nt <= 1) {
+        worker();
+    } else {
+        std::vector<std::thread> pool;
+        for (int i = 0; i < nt; i++) pool.emplace_back(worker);
+        for (auto &th : pool) th.join();
+    }
+    return EGGMAN_OK;
+}
+
+int build_plan_host(const PlanLayout &lay, void *starts, void *trans, void *levels, int nthreads) {
+    if (lay.index_bytes == 4)
+        return build_plan_t<int32_t>(lay, (int32_t *)starts, (int32_t *)trans, (int32_t *)levels,
+                                     nthreads);
+    return build_plan_t<int64_t>(lay, (int64_t *)starts, (int64_t *)trans, (int64_t *)levels,
+                                 nthreads);
+}
+
+// Check that caller-supplied arrays are a plan the kernels can walk safely: the closed-form
+// sizes and level boundaries, a uniform out-degree per level (what lets the device drop
+// `starts`), and every transition pointing into the level below with a valid pair index.
+template <typename W>
+static int validate_t(const PlanLayout &lay, const W *starts, const W *trans, const W *levels,
+                      int nthreads) {
+    for (int L = 0; L < lay.nlevels; L++)
+        if ((int64_t)levels[L] != lay.levels[L])
+            return fail(EGGMAN_EPLAN, "levels[%d] = %lld, expected %lld", L + 1,
+                        (long long)levels[L], (long long)lay.levels[L]);
+    if ((int64_t)starts[0] != 1 || (int64_t)starts[lay.nstates] != lay.ntrans + 1)
+        return fail(EGGMAN_EPLAN, "starts does not span the transitions");
+    for (int L = 1; L <= lay.K / 2 + 1; L++) {
+        const int k = std::max(0, lay.size[L - 1] - 1);
+        for (int64_t r = 0; r < lay.nst[L - 1]; r++) {
+            const int64_t s = lay.levels[L - 1] + r;     // 1-based state
+            if ((int64_t)starts[s - 1] != lay.tstart[L - 1] + r * k + 1 ||
+                (int64_t)starts[s] != lay.tstart[L - 1] + (r + 1) * k + 1)
+                return fail(EGGMAN_EPLAN, "state %lld does not have the %d transitions of its level",
+                            (long long)s, k);
+        }
+    }
+    const int64_t mask = ((int64_t)1 << lay.bits) - 1;
+    std::atomic<int> bad(0);
+    auto check = [&](int L, int64_t t0, int64_t t1) {
+        const int64_t cf = lay.levels[L - 2], cl = lay.levels[L - 1];   // child level [cf, cl)
+        for (int64_t t = t0; t < t1; t++) {
+            const int64_t w = (int64_t)trans[t];
+            const int64_t q = w & mask, c = w >> lay.bits;
+            if (q < 1 || q > lay.npairs || c < cf || c >= cl) { bad.store(1); return; }
+        }
+    };
+    if (nthreads <= 0) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());
+    std::vector<std::thread> pool;
+    for (int L = 2; L <= lay.K / 2 + 1; L++) {
+        const int64_t a = lay.tstart[L - 1], b = lay.tstart[L];
+        const int64_t per = std::max<int64_t>(1 << 20, (b - a + nthreads - 1) / nthreads);
+        for (int64_t t = a; t < b; t += per) {
+            pool.emplace_back(check, L, t, std::min(b, t + per));
+            if ((int)pool.size() >= nthreads) {
+                for (auto &th : pool) th.join();
+                pool.clear();
+            }
+        }
+    }
+    for (auto &th : pool) th.join();
+    if (bad.load()) return fail(EGGMAN_EPLAN, "a transition points outside the level below");
+    return EGGMAN_OK;
+}
+
+int validate_plan(const PlanLayout &lay, const void *starts, const void *trans, const void *levels,
+                  int nthreads) {
+    if (lay.index_bytes == 4)
+        return validate_t<int32_t>(lay, (const int32_t *)starts, (const int32_t *)trans,
+                                   (const int32_t *)levels, nthreads);
+    return validate_t<int64_t>(lay, (const int64_t *)starts, (const int64_t *)trans,
+                               (const int64_t *)levels, nthreads);
+}
+
+}  // namespace eggman

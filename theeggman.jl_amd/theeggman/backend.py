@@ -1,0 +1,149 @@
+"""The device hook: Python mirror of ``_haf_dp_backend`` and the device caches
+(src/dp.jl:514-587, ext/TheEggmanKernelAbstractionsExt.jl:28-146), implemented by the HIP
+library through its C ABI.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+
+import numpy as np
+
+from ._lib import (C32, C64, F32, F64, I128, ArgumentError, DimensionMismatch, check, lib)
+from .plan import HafnianDPPlan, _dp_counts
+
+_DTYPE_CODE = {np.dtype(np.float32): F32, np.dtype(np.float64): F64,
+               np.dtype(np.complex64): C32, np.dtype(np.complex128): C64,
+               np.dtype(np.int64): I128}
+
+
+class HIPBackend:
+    """The MI355X backend.  Any singleton works as a backend for the router, which only tests
+    ``backend is not None`` (src/hafnian.jl:1000).
+
+    ``plan_source`` chooses how a degree's plan reaches the device: ``"upload"`` copies the
+    host-built HafnianDPPlan (the reference's _device_plan, ext:31-42); ``"generate"`` builds it
+    directly in device memory, bit-identical, without a host copy.
+    """
+
+    def __init__(self, plan_source: str = "upload"):
+        if plan_source not in ("upload", "generate"):
+            raise ArgumentError(f"plan_source must be 'upload' or 'generate', got {plan_source!r}")
+        self.plan_source = plan_source
+
+    def __repr__(self):
+        return f"HIPBackend(plan_source={self.plan_source!r})"
+
+    def __eq__(self, other):
+        return isinstance(other, HIPBackend) and other.plan_source == self.plan_source
+
+    def __hash__(self):
+        return hash(("HIPBackend", self.plan_source))
+
+
+# Handles of device-resident plans keyed by (backend, K, layout), src/dp.jl:536.
+_DEVICE_PLANS: dict = {}
+_DEVICE_PLAN_LOCK = threading.RLock()
+
+
+def dtype_code(dt) -> int:
+    dt = np.dtype(dt)
+    if dt not in _DTYPE_CODE:
+        raise ArgumentError(f"element type {dt} has no device path")
+    return _DTYPE_CODE[dt]
+
+
+def _device_plan(backend: HIPBackend, plan: HafnianDPPlan) -> int:
+    """Device copy of ``plan`` (ext/TheEggmanKernelAbstractionsExt.jl:31-42), cached."""
+    key = (backend, plan.K, plan.index_bytes)
+    with _DEVICE_PLAN_LOCK:
+        h = _DEVICE_PLANS.get(key)
+        if h is None:
+            handle = ctypes.c_uint64()
+            if backend.plan_source == "generate":
+                check(lib().eggman_plan_generate(plan.K, plan.index_bytes, ctypes.byref(handle)),
+                      "plan_generate")
+            else:
+                s = np.ascontiguousarray(plan.starts)
+                t = np.ascontiguousarray(plan.trans)
+                lv = np.ascontiguousarray(plan.levels)
+                check(lib().eggman_plan_upload(plan.K, plan.index_bytes, s.ctypes.data, s.size,
+                                               t.ctypes.data, t.size, lv.ctypes.data, lv.size,
+                                               ctypes.byref(handle)), "plan_upload")
+            h = handle.value
+            _DEVICE_PLANS[key] = h
+        return h
+
+
+def _as_pair_major(P: np.ndarray) -> np.ndarray:
+    """Julia's Matrix{T}(B, npairs) is column-major, i.e. [pair][batch] with the batch index
+    fastest; a numpy (B, npairs) array in Fortran order has exactly that memory."""
+    return np.asfortranarray(P)
+
+
+def _haf_dp_backend(backend: HIPBackend, P: np.ndarray, plan: HafnianDPPlan, *,
+                    max_batch: int | None = None):
+    """Hafnians of the B instances gathered in ``P`` (shape (B, npairs)) on the device
+    (src/dp.jl:525; ext/TheEggmanKernelAbstractionsExt.jl:90-146).
+
+    Returns a length-B array of P's element type -- or, for int64 P (the exact path), a list of
+    Python ints.  ``max_batch`` bounds the instances on the device at once; results are
+    bit-identical for every value.
+    """
+    if not isinstance(backend, HIPBackend):
+        raise ArgumentError(f"no GPU backend method for {type(backend).__name__}; pass a "
+                            "theeggman.HIPBackend()")
+    P = np.asarray(P)
+    if P.ndim != 2:
+        raise DimensionMismatch(f"P must be B x npairs, got shape {P.shape}")
+    B, npairs = P.shape
+    if npairs != plan.K * (plan.K - 1) // 2:
+        raise DimensionMismatch(f"P has {npairs} pair columns, the degree-{plan.K} plan "
+                                f"expects {plan.K * (plan.K - 1) // 2}")
+    code = dtype_code(P.dtype)
+    if B == 0:
+        return [] if code == I128 else np.empty(0, P.dtype)
+    h = _device_plan(backend, plan)
+    Pf = _as_pair_major(P)
+    out = np.empty((B, 2), np.uint64) if code == I128 else np.empty(B, P.dtype)
+    mb = 0 if max_batch is None else max(1, min(B, int(max_batch)))
+    check(lib().eggman_dp_forward(h, code, Pf.ctypes.data, B, npairs, mb, out.ctypes.data),
+          "dp_forward")
+    if code == I128:
+        return [_limbs_to_int(int(lo), int(hi)) for lo, hi in out]
+    return out
+
+
+def _limbs_to_int(lo: int, hi: int) -> int:
+    v = (hi << 64) | lo
+    return v - (1 << 128) if v >> 127 else v
+
+
+def _haf_dp_device(backend: HIPBackend, plan: HafnianDPPlan, dtype, P_ptr: int, ldp: int,
+                   B: int, out_ptr: int, stream: int = 0) -> None:
+    """Device-resident forward: P (``[pair][ldp]``) and out already in device memory, enqueued on
+    ``stream`` (a raw hipStream_t, 0 = the library's stream).  Returns without synchronising."""
+    h = _device_plan(backend, plan)
+    check(lib().eggman_dp_forward_device(h, dtype_code(dtype), ctypes.c_void_p(P_ptr), ldp, B,
+                                         ctypes.c_void_p(out_ptr), ctypes.c_void_p(stream)),
+          "dp_forward_device")
+
+
+def gpu_cache_bytes() -> int:
+    """Device memory held by cached plans and work buffers (src/dp.jl:552-560)."""
+    with _DEVICE_PLAN_LOCK:
+        return int(lib().eggman_cache_bytes())
+
+
+def empty_gpu_cache() -> None:
+    """Drop every cached device plan and work buffer (src/dp.jl:567-573)."""
+    with _DEVICE_PLAN_LOCK:
+        _DEVICE_PLANS.clear()
+        check(lib().eggman_cache_clear(), "cache_clear")
+
+
+def dp_batch_bytes(N: int, T, B: int) -> int:
+    """Device memory one batch needs besides the plan: B*(nstates+npairs)*sizeof(T)
+    (src/dp.jl:584-587)."""
+    nstates, _ = _dp_counts(N)
+    return B * (nstates + N * (N - 1) // 2) * np.dtype(T).itemsize
