@@ -76,13 +76,13 @@ def cpu_baseline(K, budget_s=12.0):
     import oracle
     from theeggman import synth
     nthreads = min(os.cpu_count() or 1, oracle.max_threads())
-    Bs = max(nthreads * 8, 64)
+    Bs = max(nthreads * 32, 256)
     As = np.stack([synth.sym_normal(K, 3, b, 1 / np.sqrt(K)) for b in range(Bs)])
     P = synth.pair_batch(As)
     oracle.build_plan(K)
     oracle.haf_dp(P, K, oracle.F64, nthreads)            # warmup
     best, spent, reps = float("inf"), 0.0, 0
-    while spent < budget_s and reps < 50:
+    while spent < budget_s and reps < 100:
         t0 = time.perf_counter()
         oracle.haf_dp(P, K, oracle.F64, nthreads)
         dt = time.perf_counter() - t0
@@ -103,6 +103,9 @@ def main():
     ap.add_argument("--N", type=int, default=N_DEFAULT)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path lines")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
+                    help="after the timed steps, run one device-to-device copy of this many MiB "
+                         "(a kernel of known traffic, to scale the HBM counters of a profile)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): the batched path "
@@ -175,6 +178,14 @@ def main():
                      "alg_bytes_per_launch": nbytes / launches,
                      "avg_launch_ms": ms / launches},
     }
+
+    if args.copy_calibration > 0:
+        nbytes_copy = args.copy_calibration << 20
+        src, dst = hip.DeviceBuffer(nbytes_copy), hip.DeviceBuffer(nbytes_copy)
+        src.zero()
+        hip.copy_d2d(dst, src, nbytes_copy)
+        hip.device_synchronize()
+        result["copy_calibration_bytes"] = nbytes_copy
 
     if not args.no_plain:
         for name, T, gen in (("plain_f64", np.float64, lambda: synth.sym_normal(K, 3, 0, 1 / np.sqrt(K))),

@@ -128,6 +128,67 @@ int eggman_dp_forward(uint64_t plan, int dtype, const void *P_host, int64_t B, i
 int eggman_dp_forward_device(uint64_t plan, int dtype, const void *P_dev, int64_t ldp,
                              int64_t B, void *out_dev, void *stream);
 
+/*
+ * Hafnians of instances of different degrees in one device pass.  Group g has counts[g]
+ * instances sharing plan handle plans[g]; P_host is the concatenation of the groups'
+ * [pair][counts[g]] blocks (group 0 first) and out_host receives the groups' results in the same
+ * order.  Every launch advances each group by one level, so all degrees share the GPU at once.
+ * All plans must live on one device and use one index width.  Results are bit-identical to
+ * eggman_dp_forward on each group alone.  New: the reference requires one size per batch
+ * (src/hafnian.jl:1277-1280).
+ */
+int eggman_dp_forward_mixed(int ngroups, const uint64_t *plans, const int64_t *counts, int dtype,
+                            const void *P_host, void *out_host);
+
+/* Device-resident form of eggman_dp_forward_mixed, asynchronous on `stream`. */
+int eggman_dp_forward_mixed_device(int ngroups, const uint64_t *plans, const int64_t *counts,
+                                   int dtype, const void *P_dev, void *out_dev, void *stream);
+
+/* ---------------------------------------------------------------- reverse sweep */
+
+/*
+ * Hafnians and their gradients with respect to the pair values: out_host as for
+ * eggman_dp_forward, dP_host ([pair][batch], same layout and element size as the results)
+ * receives d haf_b / d P[q, b] for every pair q.  New: the reference has no derivative.  The
+ * caller maps dP back to the matrix through its gather (src/dp.jl:464-512): with distinct
+ * indices, d haf / d a_ij = dP[pair_index(i, j)] for the symmetric pair variable a_ij, which
+ * equals the hafnian of A with rows and columns i and j removed.  The sums are deterministic
+ * (fixed order, no atomics).  Instances are processed in chunks of at most max_batch (<= 0:
+ * as many as fit in device memory).
+ */
+int eggman_dp_forward_grad(uint64_t plan, int dtype, const void *P_host, int64_t B,
+                           int64_t npairs, int64_t max_batch, void *out_host, void *dP_host);
+
+/* Device-resident form of eggman_dp_forward_grad: dP_dev rows are lddp elements apart.
+ * Asynchronous on `stream` (NULL = the library's stream). */
+int eggman_dp_forward_grad_device(uint64_t plan, int dtype, const void *P_dev, int64_t ldp,
+                                  int64_t B, void *out_dev, void *dP_dev, int64_t lddp,
+                                  void *stream);
+
+/* ---------------------------------------------------------------- autotuner */
+
+/*
+ * Time the launch choices for (plan, dtype, B) -- lanes per state, LDS staging of the pair rows,
+ * workgroups per CU, and the instances per pass over the levels -- on synthetic pair values,
+ * `reps` runs each, and keep the fastest for this (device, degree, layout, dtype, B rounded up
+ * to a power of two).  Later calls of the forward entry points use it.  Every choice gives
+ * bit-identical results.  A text report (one line per candidate) is written to `report`.
+ * New: the reference has no tuning.
+ */
+int eggman_autotune(uint64_t plan, int dtype, int64_t B, int reps, char *report,
+                    int64_t report_len);
+
+/*
+ * Set / read a tuning entry directly (e.g. to persist tuned choices across processes).  bt:
+ * lanes per state (1..64, power of two); chunk: instances per pass; lds: stage pair rows in LDS;
+ * occ: workgroups per CU (0 = as LDS allows); graph: replay the level sequence from a HIP graph
+ * captured on first use (reused while the same buffers are passed).
+ */
+int eggman_tune_set(int K, int index_bytes, int dtype, int64_t B, int bt, int64_t chunk, int lds,
+                    int occ, int graph);
+int eggman_tune_get(int K, int index_bytes, int dtype, int64_t B, int *bt, int64_t *chunk,
+                    int *lds, int *occ, int *graph);
+
 /* ---------------------------------------------------------------- caches */
 
 /* Device bytes held by cached plans and work buffers: the library's share of gpu_cache_bytes

@@ -1,0 +1,152 @@
+"""The reverse sweep (d haf / d A) on an MI355X, checked against what the mathematics fixes.
+
+* Deletion identity: d haf / d a_ij = haf(A with rows and columns i, j removed), evaluated by the
+  CPU oracle at degree N-2 -- exactly, on the 128-bit integer path.
+* Central differences with h = 1e-6.  The hafnian is affine in each symmetric pair variable, so
+  the central difference has no truncation error: it equals the derivative up to the rounding of
+  the two forward values, whose size is measured against the __float128 oracle and sets the
+  tolerance.
+* Homogeneity (sum_{i<j} a_ij d haf/d a_ij = (N/2) haf) and the fold of a repeated-row gradient.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as o
+import theeggman as te
+from theeggman import _lib, synth
+
+pytestmark = pytest.mark.gpu
+BE = te.HIPBackend()
+
+
+def note(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(autouse=True)
+def _gpu_present():
+    if _lib.lib().eggman_device_count() < 1:
+        pytest.skip("no GPU visible")
+
+
+def deletion(A, i, j, dtype):
+    keep = [k for k in range(A.shape[0]) if k not in (i, j)]
+    Ar = A[np.ix_(keep, keep)]
+    return o.haf_dp(o.gather_pairs(Ar).reshape(-1, 1), len(keep), dtype, 8)[0]
+
+
+@pytest.mark.parametrize("N", [8, 12, 16])
+def test_exact_gradient_is_the_deletion_hafnian(N):
+    note(f"exact grad N={N}")
+    A = np.random.default_rng(N).integers(-3, 4, (N, N)).astype(np.int64)
+    A = np.triu(A, 1)
+    A = A + A.T
+    P = te._gather_pairs_batch(np.int64, A, None, N)
+    vals, dP = te._haf_dp_backend_grad(BE, P, te._dp_plan(N))
+    assert vals[0] == o.haf_dp(o.gather_pairs(A).reshape(-1, 1), N, o.I128)[0]
+    q = 0
+    for i in range(N):
+        for j in range(i + 1, N):
+            assert dP[0][q] == deletion(A, i, j, o.I128), (i, j)
+            q += 1
+    # homogeneity, exactly
+    assert sum(int(A[i, j]) * dP[0][te._pair_index(i + 1, j + 1, N) - 1]
+               for i in range(N) for j in range(i + 1, N)) * 2 == N * vals[0]
+
+
+@pytest.mark.parametrize("T,N", [(np.float64, 16), (np.float64, 20), (np.complex128, 16)])
+def test_float_gradient_matches_deletion_identity(T, N):
+    note(f"grad {np.dtype(T)} N={N}")
+    A = (synth.sym_complex_normal(N, 30, 0) if T == np.complex128 else synth.sym_uniform(N, 30, 0))
+    val, G = te.hafnian_grad(A.astype(T))
+    assert val == te.hafnian(A.astype(T))                    # same forward, bit for bit
+    D = np.zeros((N, N), T)
+    for i in range(N):
+        for j in range(i + 1, N):
+            D[i, j] = D[j, i] = deletion(A, i, j, o.C64 if T == np.complex128 else o.F64)
+    scale = np.abs(D).max()
+    err = np.abs(G - D).max() / scale
+    note(f"  max |G - deletion| / max|deletion| = {err:.2e}")
+    assert err < 1e-12
+    assert (np.diag(G) == 0).all()
+    assert np.allclose(G, G.T, rtol=0, atol=0)
+    lhs = np.sum(np.triu(A, 1) * np.triu(G, 1))
+    assert abs(lhs - N / 2 * val) <= 1e-11 * max(1.0, abs(N / 2 * val), np.abs(A).sum() * scale)
+
+
+def test_config4_batched_gradient_n24():
+    # 256 real Float64 N=24 matrices, Uniform(-1, 1): full gradient, checked on a sample against
+    # the deletion identity and central differences (h = 1e-6)
+    note("config 4: N=24 B=256 forward+gradient")
+    N, B, h = 24, 256, 1e-6
+    As = np.stack([synth.sym_uniform(N, 4, b) for b in range(B)])
+    vals, Gs = te.hafnian_grad(As, check_symmetric=False)
+    assert vals.tobytes() == te.hafnian(As, check_symmetric=False).tobytes()
+    rng = np.random.default_rng(4)
+    pairs = [(0, 1), (0, 23), (5, 17), (11, 12), (22, 23)]
+    for b in (0, 100, 255):
+        A = As[b]
+        for (i, j) in pairs:
+            d = deletion(A, i, j, o.F64)
+            assert abs(Gs[b, i, j] - d) <= 1e-12 * max(abs(d), np.abs(Gs[b]).max() * 1e-3)
+        # central differences, batched: A +- h E_ij for each sampled pair
+        Ap, Am = [], []
+        for (i, j) in pairs:
+            E = np.zeros((N, N))
+            E[i, j] = E[j, i] = h
+            Ap.append(A + E)
+            Am.append(A - E)
+        fp = te.hafnian(np.stack(Ap), check_symmetric=False)
+        fm = te.hafnian(np.stack(Am), check_symmetric=False)
+        # rounding of the forward values, measured against the __float128 oracle
+        qp = o.haf_dp(synth.pair_batch(np.stack(Ap)), N, o.Q, 8)
+        qm = o.haf_dp(synth.pair_batch(np.stack(Am)), N, o.Q, 8)
+        for k, (i, j) in enumerate(pairs):
+            fd = (fp[k] - fm[k]) / (2 * h)
+            exact_fd = ((qp[k, 0] - qm[k, 0]) + (qp[k, 1] - qm[k, 1])) / (2 * h)
+            rounding = (abs(fp[k] - qp[k, 0] - qp[k, 1]) + abs(fm[k] - qm[k, 0] - qm[k, 1])) / (2 * h)
+            assert abs(exact_fd - Gs[b, i, j]) <= 1e-9 * max(1.0, abs(Gs[b, i, j]))
+            assert abs(fd - Gs[b, i, j]) <= 4 * rounding + 1e-9 * max(1.0, abs(Gs[b, i, j]))
+    _ = rng
+
+
+def test_gradient_is_deterministic_and_chunk_independent():
+    N, B = 16, 70
+    As = np.stack([synth.sym_uniform(N, 31, b) for b in range(B)])
+    P = te._gather_pairs_batch(np.float64, As, None, N)
+    v1, g1 = te._haf_dp_backend_grad(BE, P, te._dp_plan(N))
+    v2, g2 = te._haf_dp_backend_grad(BE, P, te._dp_plan(N))
+    v3, g3 = te._haf_dp_backend_grad(BE, P, te._dp_plan(N), max_batch=7)
+    assert v1.tobytes() == v2.tobytes() == v3.tobytes()
+    assert np.asarray(g1).tobytes() == np.asarray(g2).tobytes() == np.asarray(g3).tobytes()
+
+
+def test_repeated_gradient_folds_the_expanded_gradient():
+    A = synth.sym_uniform(6, 32, 0)
+    np.fill_diagonal(A, synth.uniform(synth.stream(32, 1), 6))
+    rpt = [2, 1, 3, 0, 1, 1]
+    val, G = te.hafnian_repeated_grad(A, rpt)
+    R = te.reduction(A, rpt)
+    vR, GR = te.hafnian_grad(R)
+    assert val == vR
+    idx = np.repeat(np.arange(6), rpt)
+    F = np.zeros((6, 6))
+    n = len(idx)
+    for a in range(n):
+        for b in range(a + 1, n):
+            u, v = idx[a], idx[b]
+            F[u, v] += GR[a, b]
+            if u != v:
+                F[v, u] += GR[a, b]
+    assert np.allclose(F, G, rtol=1e-13, atol=1e-13)
+
+
+def test_rrule_pullback():
+    A = synth.sym_uniform(12, 33, 0)
+    y, pb = te.rrule(te.hafnian, A)
+    _, G = te.hafnian_grad(A)
+    assert y == te.hafnian(A)
+    none, dA = pb(2.5)
+    assert none is None and np.array_equal(dA, 2.5 * G)

@@ -34,10 +34,6 @@ int fail(int code, const char *fmt, ...);
 // _dp_states! (src/dp.jl:344-363) bit for bit when accumulated in plan order.  Complex multiply
 // is (ac - bd, ad + bc), Julia's `*` for Complex.
 
-struct c64 { double re, im; };
-struct c32 { float re, im; };
-struct u128v { uint64_t lo, hi; };   // two's-complement 128-bit value, wrapping
-
 template <int DT> struct Elt;
 
 template <> struct Elt<EGGMAN_F64> {
@@ -46,6 +42,8 @@ template <> struct Elt<EGGMAN_F64> {
     __device__ static inline H zero() { return 0.0; }
     __device__ static inline H one() { return 1.0; }
     __device__ static inline H muladd(H acc, P p, H h) { return acc + p * h; }
+    __device__ static inline H muladd_hh(H acc, H a, H b) { return acc + a * b; }
+    __device__ static inline H add_h(H a, H b) { return a + b; }
 };
 template <> struct Elt<EGGMAN_F32> {
     using H = float;
@@ -53,6 +51,8 @@ template <> struct Elt<EGGMAN_F32> {
     __device__ static inline H zero() { return 0.0f; }
     __device__ static inline H one() { return 1.0f; }
     __device__ static inline H muladd(H acc, P p, H h) { return acc + p * h; }
+    __device__ static inline H muladd_hh(H acc, H a, H b) { return acc + a * b; }
+    __device__ static inline H add_h(H a, H b) { return a + b; }
 };
 template <> struct Elt<EGGMAN_C64> {
     using H = double2;
@@ -64,6 +64,8 @@ template <> struct Elt<EGGMAN_C64> {
         const double im = p.x * h.y + p.y * h.x;
         return make_double2(acc.x + re, acc.y + im);
     }
+    __device__ static inline H muladd_hh(H acc, H a, H b) { return muladd(acc, a, b); }
+    __device__ static inline H add_h(H a, H b) { return make_double2(a.x + b.x, a.y + b.y); }
 };
 template <> struct Elt<EGGMAN_C32> {
     using H = float2;
@@ -75,6 +77,8 @@ template <> struct Elt<EGGMAN_C32> {
         const float im = p.x * h.y + p.y * h.x;
         return make_float2(acc.x + re, acc.y + im);
     }
+    __device__ static inline H muladd_hh(H acc, H a, H b) { return muladd(acc, a, b); }
+    __device__ static inline H add_h(H a, H b) { return make_float2(a.x + b.x, a.y + b.y); }
 };
 // Exact integer walk: states are 128-bit integers modulo 2^128, pair values are int64.  The
 // DP only adds and multiplies, so the result is exact whenever the true hafnian fits in 127
@@ -87,12 +91,20 @@ template <> struct Elt<EGGMAN_I128> {
     __device__ static inline H muladd(H acc, P p, H h) {
         const unsigned long long pl = (unsigned long long)p;
         const unsigned long long ph = p < 0 ? ~0ull : 0ull;     // sign extension
-        const unsigned long long lo = pl * h.x;
-        const unsigned long long hi = __umul64hi(pl, h.x) + pl * h.y + ph * h.x;
-        const unsigned long long rlo = acc.x + lo;
-        const unsigned long long rhi = acc.y + hi + (rlo < lo ? 1ull : 0ull);
-        return make_ulonglong2(rlo, rhi);
+        return add(acc, mul(make_ulonglong2(pl, ph), h));
     }
+    // a * b modulo 2^128
+    __device__ static inline H mul(H a, H b) {
+        const unsigned long long lo = a.x * b.x;
+        const unsigned long long hi = __umul64hi(a.x, b.x) + a.x * b.y + a.y * b.x;
+        return make_ulonglong2(lo, hi);
+    }
+    __device__ static inline H add(H a, H b) {
+        const unsigned long long lo = a.x + b.x;
+        return make_ulonglong2(lo, a.y + b.y + (lo < a.x ? 1ull : 0ull));
+    }
+    __device__ static inline H muladd_hh(H acc, H a, H b) { return add(acc, mul(a, b)); }
+    __device__ static inline H add_h(H a, H b) { return add(a, b); }
 };
 
 inline int elt_hbytes(int dtype) {

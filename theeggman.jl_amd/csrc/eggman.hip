@@ -14,6 +14,7 @@
 #include "common.hpp"
 #include "kernels_fwd.hpp"
 #include "kernels_plan.hpp"
+#include "kernels_rev.hpp"
 #include "plan_host.hpp"
 
 #define EGGMAN_VERSION_STRING "eggman-hip 0.1 (gfx950)"
@@ -44,6 +45,13 @@ struct DevPlan {
     PlanLayout lay;
     void *d_trans = nullptr;
     size_t bytes = 0;
+    // reverse-sweep index structures (kernels_rev.hpp), built on the first gradient call
+    bool rev_ready = false;
+    void *d_cent = nullptr, *d_coff = nullptr, *d_pent = nullptr, *d_segs = nullptr;
+    void *d_qoff = nullptr, *d_qseg = nullptr;
+    std::vector<int64_t> lvl_seg;
+    int64_t nseg = 0;
+    size_t rev_bytes = 0;
 };
 
 // Grow-only device buffer (ext/TheEggmanKernelAbstractionsExt.jl:68-88).
@@ -52,13 +60,49 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// What the launch heuristics need to know about a device.
+struct DevCaps {
+    int cus = 256;                   // compute units
+    int xcds = 8;                    // chiplets, each with its own L2
+    size_t max_lds_block = 65536;    // LDS one workgroup may use
+    size_t lds_per_cu = 65536;       // LDS per compute unit
+};
+
+// A captured forward walk and the arguments it was captured for.
+struct GraphEntry {
+    uint64_t plan = 0;
+    int dtype = 0;
+    const void *P = nullptr;
+    int64_t ldp = 0, B = 0;
+    void *out = nullptr;
+    int bt = 0;
+    int64_t chunk = 0;
+    bool lds = false;
+    int occ = 0;
+    uint64_t gen = 0;                // work-buffer generation the graph's addresses belong to
+    hipGraphExec_t exec = nullptr;
+};
+
+// Bumped whenever a work buffer is reallocated, which invalidates captured graphs.
+static uint64_t g_buf_gen = 1;
+
 struct DevState {
+    DevCaps caps;
+    hipStream_t cap = nullptr;       // capture stream for graphs
+    std::vector<GraphEntry> graphs;
     hipStream_t stream = nullptr;    // the library's own stream on this device
     hipStream_t last = nullptr;      // stream of the most recent enqueued call
     hipEvent_t order = nullptr;      // orders calls on different streams (shared buffers)
     DevBuf H;                        // state values (ping-pong levels)
     DevBuf P;                        // staged pair values (host entry points)
     DevBuf out;                      // staged results (host entry points)
+    DevBuf Hg;                       // every level of H, for the reverse sweep
+    DevBuf G;                        // adjoint values (ping-pong levels)
+    DevBuf part;                     // per-segment partial gradients
+    DevBuf dP;                       // staged gradients (host entry points)
+    size_t bytes() const {
+        return H.bytes + P.bytes + out.bytes + Hg.bytes + G.bytes + part.bytes + dP.bytes;
+    }
 };
 
 static std::recursive_mutex g_mu;
@@ -75,6 +119,15 @@ static int current_device(int *dev) {
 static int dev_state(int device, DevState **ds) {
     DevState &d = g_dev[device];
     if (!d.stream) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && v > 0)
+            d.caps.cus = v;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && v > 0)
+            d.caps.max_lds_block = (size_t)v;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) == hipSuccess && v > 0)
+            d.caps.lds_per_cu = (size_t)v;
+        else
+            d.caps.lds_per_cu = d.caps.max_lds_block;
         EGG_HIP(hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking));
         EGG_HIP(hipEventCreateWithFlags(&d.order, hipEventDisableTiming));
         d.last = d.stream;
@@ -96,6 +149,7 @@ static int chain_stream(DevState &d, hipStream_t s) {
 
 static int ensure(DevBuf &b, size_t bytes) {
     if (b.bytes >= bytes) return EGGMAN_OK;
+    g_buf_gen++;
     if (b.p) {
         EGG_HIP(hipDeviceSynchronize());
         EGG_HIP(hipFree(b.p));
@@ -142,6 +196,9 @@ struct DeviceGuard {
 struct Tune {
     int bt = 64;        // lanes per state (batch tile width), power of two in [1, 64]
     int64_t chunk = 0;  // instances per pass over the levels (0 = all)
+    bool lds = true;    // stage the level's pair rows in LDS (bt == 64 only)
+    int occ = 0;        // workgroups per CU for the LDS kernel (0 = as many as LDS allows, <= 4)
+    bool graph = false; // replay the level sequence from a captured HIP graph
 };
 
 static int pow2_at_least(int64_t n) {
@@ -154,6 +211,26 @@ static Tune default_tune(const PlanLayout &, int, int64_t B) {
     Tune t;
     t.bt = pow2_at_least(B);
     t.chunk = B;
+    return t;
+}
+
+// Tuned choices per (device, degree, index width, element type, batch rounded up to a power of
+// two), written by eggman_autotune or eggman_tune_set.
+using TuneKey = std::tuple<int, int, int, int, int64_t>;
+static std::map<TuneKey, Tune> g_tune;
+
+static int64_t batch_bucket(int64_t B) {
+    int64_t b = 1;
+    while (b < B) b <<= 1;
+    return b;
+}
+
+static Tune pick_tune(const DevPlan &pl, int dtype, int64_t B) {
+    auto it = g_tune.find(TuneKey(pl.device, pl.lay.K, pl.lay.index_bytes, dtype, batch_bucket(B)));
+    if (it == g_tune.end()) return default_tune(pl.lay, dtype, B);
+    Tune t = it->second;
+    if (t.bt > pow2_at_least(B) && t.bt != 64) t.bt = pow2_at_least(B);
+    if (t.chunk <= 0 || t.chunk > B) t.chunk = B;
     return t;
 }
 
@@ -173,6 +250,85 @@ static void launch_level(const DevPlan &pl, int L, const void *P, int64_t ldp, c
     k_level_fwd<DT, W, BT><<<dim3((unsigned)blocks), dim3(kBlock), 0, st>>>(
         tl, nst, k, (const typename E::P *)P, ldp, (const typename E::H *)Hc, lay.levels[L - 2],
         ldh, (typename E::H *)Ho, ldo, nb, ntiles);
+}
+
+constexpr int kLdsWaves = 8;   // waves per workgroup of the LDS kernel
+
+// Workgroup-to-(tile, states) mapping of the LDS kernels for `nst` states of `nb` instances:
+// enough workgroups per XCD to fill its CUs at the occupancy the LDS use allows, and no more
+// than there are states.
+static LdsMap make_lds_map(const DevCaps &caps, int occ_cap, int64_t nst, int64_t nb, size_t lds) {
+    int occ = (int)std::max<size_t>(1, std::min<size_t>(4, caps.lds_per_cu / std::max<size_t>(lds, 1)));
+    if (occ_cap > 0) occ = std::min(occ, occ_cap);
+    const int cus_xcd = std::max(1, caps.cus / caps.xcds);
+    const int64_t target = (int64_t)cus_xcd * occ;
+    LdsMap map;
+    map.ntiles = (nb + 63) / 64;
+    if (map.ntiles >= 8) {
+        map.tiles_xcd = (map.ntiles + 7) / 8;
+        int64_t bpt = std::max<int64_t>(1, target / map.tiles_xcd);
+        bpt = std::min<int64_t>(bpt, std::max<int64_t>(1, (nst + kLdsWaves - 1) / kLdsWaves));
+        map.per_xcd = map.tiles_xcd * bpt;
+    } else {
+        map.tiles_xcd = 0;
+        const int64_t groups = (8 + map.ntiles - 1) / map.ntiles;
+        const int64_t per_group = (nst + groups - 1) / groups;
+        map.per_xcd = std::max<int64_t>(1, std::min<int64_t>(target, (per_group + kLdsWaves - 1) / kLdsWaves));
+    }
+    return map;
+}
+
+// The contiguous range of pair indices read by level L: rows p_lo .. p_hi of the upper
+// triangle, where p = min(S) of the level's states (plan_rank.hpp).  qlo is 0-based.
+static void level_pairs(const PlanLayout &lay, int L, int *qlo, int *nq) {
+    const int K = lay.K, m = lay.size[L - 1];
+    const int plo = lay.lo[L - 1];
+    const int phi = std::min(K - m + 1, K - 1);
+    *qlo = (int)pair_index(plo, plo + 1, K) - 1;
+    *nq = (int)pair_index(phi, K, K) - 1 - *qlo + 1;
+}
+
+// Launch one level with the LDS kernel; returns false (launching nothing) when the level's pair
+// rows do not fit in one workgroup's LDS.
+template <int DT, typename W>
+static bool launch_level_lds(const DevCaps &caps, const Tune &tune, const DevPlan &pl, int L,
+                             const void *P, int64_t ldp, const void *Hc, int64_t ldh, void *Ho,
+                             int64_t ldo, int64_t nb, hipStream_t st) {
+    using E = Elt<DT>;
+    const PlanLayout &lay = pl.lay;
+    int qlo, nq;
+    level_pairs(lay, L, &qlo, &nq);
+    const size_t lds = (size_t)nq * 64 * sizeof(typename E::P);
+    if (lds > caps.max_lds_block) return false;
+    auto kern = k_level_fwd_lds<DT, W, kLdsWaves>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)caps.max_lds_block) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        attr_set = true;
+    }
+    const int64_t nst = lay.nst[L - 1];
+    const LdsMap map = make_lds_map(caps, tune.occ, nst, nb, lds);
+    LevelArgs<DT, W> a;
+    a.trans_lvl = (const W *)pl.d_trans + lay.tstart[L - 1];
+    a.nst = nst;
+    a.k = lay.size[L - 1] - 1;
+    a.qlo = qlo;
+    a.nq = nq;
+    a.P = (const typename E::P *)P;
+    a.ldp = ldp;
+    a.Hc = (const typename E::H *)Hc;
+    a.cfirst = lay.levels[L - 2];
+    a.ldh = ldh;
+    a.Ho = (typename E::H *)Ho;
+    a.ldo = ldo;
+    a.nb = nb;
+    a.map = map;
+    kern<<<dim3((unsigned)(8 * map.per_xcd)), dim3(kLdsWaves * 64), lds, st>>>(a);
+    return true;
 }
 
 template <int DT, typename W>
@@ -213,6 +369,9 @@ static int forward_levels(DevState &ds, const DevPlan &pl, const void *P, int64_
         for (int L = 2; L <= top; L++) {
             const void *Hc = buf[(L - 1) & 1];
             void *Ho = L == top ? (void *)outc : (void *)buf[L & 1];
+            if (tune.bt == 64 && tune.lds &&
+                launch_level_lds<DT, W>(ds.caps, tune, pl, L, Pc, ldp, Hc, nb, Ho, nb, nb, st))
+                continue;
             launch_level_bt<DT, W>(tune.bt, pl, L, Pc, ldp, Hc, nb, Ho, nb, nb, st);
         }
     }
@@ -227,8 +386,8 @@ static int forward_dt(DevState &ds, const DevPlan &pl, const void *P, int64_t ld
     return forward_levels<DT, int64_t>(ds, pl, P, ldp, B, out, st, tune);
 }
 
-static int forward_any(int dtype, DevState &ds, const DevPlan &pl, const void *P, int64_t ldp,
-                       int64_t B, void *out, hipStream_t st, const Tune &tune) {
+static int forward_eager(int dtype, DevState &ds, const DevPlan &pl, const void *P, int64_t ldp,
+                         int64_t B, void *out, hipStream_t st, const Tune &tune) {
     switch (dtype) {
     case EGGMAN_F64: return forward_dt<EGGMAN_F64>(ds, pl, P, ldp, B, out, st, tune);
     case EGGMAN_F32: return forward_dt<EGGMAN_F32>(ds, pl, P, ldp, B, out, st, tune);
@@ -237,6 +396,378 @@ static int forward_any(int dtype, DevState &ds, const DevPlan &pl, const void *P
     case EGGMAN_I128: return forward_dt<EGGMAN_I128>(ds, pl, P, ldp, B, out, st, tune);
     }
     return fail(EGGMAN_EINVAL, "unknown dtype %d", dtype);
+}
+
+// The level sequence of one forward call captured once into a HIP graph and replayed: a call
+// repeated with the same buffers (a serving loop, the plain path) then costs one graph launch
+// instead of K/2 + 1 kernel launches from the host.  Graphs hold the library's work-buffer
+// addresses, so a graph is dropped whenever a work buffer is reallocated.
+static int forward_graph(int dtype, DevState &ds, const DevPlan &pl, const void *P, int64_t ldp,
+                         int64_t B, void *out, hipStream_t st, const Tune &tune) {
+    for (auto it = ds.graphs.begin(); it != ds.graphs.end(); ++it) {
+        if (it->gen != g_buf_gen) continue;
+        if (it->plan == pl.id && it->dtype == dtype && it->P == P && it->ldp == ldp && it->B == B &&
+            it->out == out && it->bt == tune.bt && it->chunk == tune.chunk && it->lds == tune.lds &&
+            it->occ == tune.occ) {
+            EGG_HIP(hipGraphLaunch(it->exec, st));
+            return EGGMAN_OK;
+        }
+    }
+    // size the work buffers outside the capture (an eager pass also validates the launch)
+    EGG_TRY(forward_eager(dtype, ds, pl, P, ldp, B, out, st, tune));
+    if (!ds.cap) EGG_HIP(hipStreamCreateWithFlags(&ds.cap, hipStreamNonBlocking));
+    EGG_HIP(hipStreamBeginCapture(ds.cap, hipStreamCaptureModeThreadLocal));
+    const int rc = forward_eager(dtype, ds, pl, P, ldp, B, out, ds.cap, tune);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(ds.cap, &graph);
+    if (rc != EGGMAN_OK || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        return rc != EGGMAN_OK ? rc : fail(EGGMAN_EHIP, "graph capture failed: %s", hipGetErrorString(e));
+    }
+    GraphEntry ge;
+    ge.plan = pl.id;
+    ge.dtype = dtype;
+    ge.P = P;
+    ge.ldp = ldp;
+    ge.B = B;
+    ge.out = out;
+    ge.bt = tune.bt;
+    ge.chunk = tune.chunk;
+    ge.lds = tune.lds;
+    ge.occ = tune.occ;
+    ge.gen = g_buf_gen;
+    const hipError_t ei = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return fail(EGGMAN_EHIP, "graph instantiation failed: %s", hipGetErrorString(ei));
+    if (ds.graphs.size() >= 16) {
+        (void)hipGraphExecDestroy(ds.graphs.front().exec);
+        ds.graphs.erase(ds.graphs.begin());
+    }
+    ds.graphs.push_back(ge);
+    return EGGMAN_OK;   // this call's results came from the eager pass
+}
+
+static int forward_any(int dtype, DevState &ds, const DevPlan &pl, const void *P, int64_t ldp,
+                       int64_t B, void *out, hipStream_t st, const Tune &tune) {
+    if (tune.graph) return forward_graph(dtype, ds, pl, P, ldp, B, out, st, tune);
+    return forward_eager(dtype, ds, pl, P, ldp, B, out, st, tune);
+}
+
+// ------------------------------------------------------------------------------ mixed degrees
+
+// Groups of instances with different degrees (and so different plans) walked together: launch t
+// advances every group by one level (level t + 2 of each group that still has levels left), so
+// one launch carries the work of all degrees and small levels of one degree ride along with large
+// levels of another.  Each group keeps its own two-level state buffers; its last level writes its
+// results.  P is the concatenation of the groups' [pair][count] blocks, out of their results.
+template <int DT, typename W>
+static int forward_mixed_levels(DevState &ds, const std::vector<DevPlan *> &pls,
+                                const std::vector<int64_t> &counts, const void *P, void *out,
+                                hipStream_t st, const Tune &tune) {
+    using E = Elt<DT>;
+    using HT = typename E::H;
+    using PT = typename E::P;
+    const size_t ng = pls.size();
+    std::vector<const PT *> Pg(ng);
+    std::vector<HT *> outg(ng), buf0(ng), buf1(ng);
+    size_t poff = 0, ooff = 0, hoff = 0;
+    for (size_t g = 0; g < ng; g++) {
+        Pg[g] = (const PT *)P + poff;
+        outg[g] = (HT *)out + ooff;
+        poff += (size_t)pls[g]->lay.npairs * counts[g];
+        ooff += (size_t)counts[g];
+        hoff += (size_t)2 * pls[g]->lay.max_level * counts[g];
+    }
+    EGG_TRY(ensure(ds.H, hoff * sizeof(HT)));
+    hoff = 0;
+    int steps = 0;
+    for (size_t g = 0; g < ng; g++) {
+        const size_t rows = (size_t)pls[g]->lay.max_level * counts[g];
+        buf0[g] = (HT *)ds.H.p + hoff;
+        buf1[g] = buf0[g] + rows;
+        hoff += 2 * rows;
+        steps = std::max(steps, pls[g]->lay.K / 2);
+        if (counts[g] > 0)
+            k_fill_one<DT><<<dim3((unsigned)((counts[g] + 255) / 256)), dim3(256), 0, st>>>(buf1[g], counts[g]);
+    }
+    auto kern = k_level_fwd_lds_multi<DT, W, kLdsWaves>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        attr_set = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)ds.caps.max_lds_block) == hipSuccess;
+        (void)hipGetLastError();
+    }
+    for (int t = 0; t < steps; t++) {
+        const int L = t + 2;
+        MultiLevelArgs<DT, W> m;
+        m.n = 0;
+        m.first[0] = 0;
+        size_t lds_max = 0;
+        auto flush = [&]() {
+            if (m.n == 0) return;
+            kern<<<dim3((unsigned)m.first[m.n]), dim3(kLdsWaves * 64), lds_max, st>>>(m);
+            m.n = 0;
+            lds_max = 0;
+        };
+        for (size_t g = 0; g < ng; g++) {
+            const DevPlan &pl = *pls[g];
+            const PlanLayout &lay = pl.lay;
+            const int64_t nb = counts[g];
+            if (nb == 0 || L > lay.K / 2 + 1) continue;
+            HT *Hc = (L - 1) & 1 ? buf1[g] : buf0[g];
+            HT *Ho = L == lay.K / 2 + 1 ? outg[g] : (L & 1 ? buf1[g] : buf0[g]);
+            int qlo, nq;
+            level_pairs(lay, L, &qlo, &nq);
+            const size_t lds = (size_t)nq * 64 * sizeof(PT);
+            if (!attr_set || !tune.lds || lds > ds.caps.max_lds_block) {
+                launch_level_bt<DT, W>(pow2_at_least(nb), pl, L, Pg[g], nb, Hc, nb, Ho, nb, nb, st);
+                continue;
+            }
+            LevelArgs<DT, W> &a = m.g[m.n];
+            a.trans_lvl = (const W *)pl.d_trans + lay.tstart[L - 1];
+            a.nst = lay.nst[L - 1];
+            a.k = lay.size[L - 1] - 1;
+            a.qlo = qlo;
+            a.nq = nq;
+            a.P = Pg[g];
+            a.ldp = nb;
+            a.Hc = Hc;
+            a.cfirst = lay.levels[L - 2];
+            a.ldh = nb;
+            a.Ho = Ho;
+            a.ldo = nb;
+            a.nb = nb;
+            a.map = make_lds_map(ds.caps, tune.occ, a.nst, nb, lds);
+            m.first[m.n + 1] = m.first[m.n] + 8 * a.map.per_xcd;
+            m.n++;
+            lds_max = std::max(lds_max, lds);
+            if (m.n == kMaxGroups) flush();
+        }
+        flush();
+    }
+    EGG_HIP(hipGetLastError());
+    return EGGMAN_OK;
+}
+
+template <int DT>
+static int forward_mixed_dt(DevState &ds, const std::vector<DevPlan *> &pls,
+                            const std::vector<int64_t> &counts, const void *P, void *out,
+                            hipStream_t st, const Tune &tune) {
+    if (pls[0]->lay.index_bytes == 4)
+        return forward_mixed_levels<DT, int32_t>(ds, pls, counts, P, out, st, tune);
+    return forward_mixed_levels<DT, int64_t>(ds, pls, counts, P, out, st, tune);
+}
+
+static int forward_mixed_any(int dtype, DevState &ds, const std::vector<DevPlan *> &pls,
+                             const std::vector<int64_t> &counts, const void *P, void *out,
+                             hipStream_t st, const Tune &tune) {
+    switch (dtype) {
+    case EGGMAN_F64: return forward_mixed_dt<EGGMAN_F64>(ds, pls, counts, P, out, st, tune);
+    case EGGMAN_F32: return forward_mixed_dt<EGGMAN_F32>(ds, pls, counts, P, out, st, tune);
+    case EGGMAN_C64: return forward_mixed_dt<EGGMAN_C64>(ds, pls, counts, P, out, st, tune);
+    case EGGMAN_C32: return forward_mixed_dt<EGGMAN_C32>(ds, pls, counts, P, out, st, tune);
+    case EGGMAN_I128: return forward_mixed_dt<EGGMAN_I128>(ds, pls, counts, P, out, st, tune);
+    }
+    return fail(EGGMAN_EINVAL, "unknown dtype %d", dtype);
+}
+
+// Validate a mixed request: known handles on one device with one index width.
+static int mixed_plans(int ngroups, const uint64_t *plans, const int64_t *counts,
+                       std::vector<DevPlan *> *pls, std::vector<int64_t> *cnt, int64_t *npairs_total,
+                       int64_t *total) {
+    if (ngroups < 1 || !plans || !counts) return fail(EGGMAN_EINVAL, "need at least one group");
+    *npairs_total = 0;
+    *total = 0;
+    for (int g = 0; g < ngroups; g++) {
+        DevPlan *pl;
+        EGG_TRY(find_plan(plans[g], &pl));
+        if (counts[g] < 0) return fail(EGGMAN_EINVAL, "negative count in group %d", g);
+        if (g > 0 && (pl->device != (*pls)[0]->device || pl->lay.index_bytes != (*pls)[0]->lay.index_bytes))
+            return fail(EGGMAN_EINVAL, "all groups of a mixed call must share a device and an index width");
+        pls->push_back(pl);
+        cnt->push_back(counts[g]);
+        *npairs_total += pl->lay.npairs * counts[g];
+        *total += counts[g];
+    }
+    return EGGMAN_OK;
+}
+
+// ------------------------------------------------------------------------------ reverse sweep
+
+constexpr int kRevSegLen = 1024;   // transitions per dP segment
+
+static int upload_vec(void **dst, const void *src, size_t bytes, size_t *acc) {
+    hipError_t e = hipMalloc(dst, std::max<size_t>(bytes, 16));
+    if (e != hipSuccess) {
+        *dst = nullptr;
+        (void)hipGetLastError();
+        return fail(EGGMAN_ENOMEM, "device allocation of %.3f GB for the reverse sweep failed",
+                    bytes / 1e9);
+    }
+    *acc += bytes;
+    if (bytes) EGG_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    return EGGMAN_OK;
+}
+
+// Build the reverse-sweep structures from the plan's own device transitions.
+static int ensure_rev(DevPlan &pl) {
+    if (pl.rev_ready) return EGGMAN_OK;
+    const PlanLayout &lay = pl.lay;
+    std::vector<uint8_t> trans((size_t)lay.ntrans * lay.index_bytes);
+    EGG_HIP(hipDeviceSynchronize());
+    EGG_HIP(hipMemcpy(trans.data(), pl.d_trans, trans.size(), hipMemcpyDeviceToHost));
+    RevPlanHost rp;
+    EGG_TRY(build_rev_plan(lay, trans.data(), kRevSegLen, &rp));
+    std::vector<RevSegDev> segs(rp.segs.size());
+    for (size_t i = 0; i < segs.size(); i++)
+        segs[i] = RevSegDev{rp.segs[i].start, rp.segs[i].len, rp.segs[i].level};
+    size_t acc = 0;
+    EGG_TRY(upload_vec(&pl.d_cent, rp.cent.data(), rp.cent.size(), &acc));
+    EGG_TRY(upload_vec(&pl.d_coff, rp.coff.data(), rp.coff.size() * sizeof(int64_t), &acc));
+    EGG_TRY(upload_vec(&pl.d_pent, rp.pent.data(), rp.pent.size() * sizeof(uint64_t), &acc));
+    EGG_TRY(upload_vec(&pl.d_segs, segs.data(), segs.size() * sizeof(RevSegDev), &acc));
+    EGG_TRY(upload_vec(&pl.d_qoff, rp.qoff.data(), rp.qoff.size() * sizeof(int64_t), &acc));
+    EGG_TRY(upload_vec(&pl.d_qseg, rp.qseg.data(), rp.qseg.size() * sizeof(int32_t), &acc));
+    pl.lvl_seg = rp.lvl_seg;
+    pl.nseg = (int64_t)rp.segs.size();
+    pl.rev_bytes = acc;
+    pl.rev_ready = true;
+    return EGGMAN_OK;
+}
+
+// Device bytes one instance needs for forward + reverse: every level of H, two levels of G,
+// the segment partials, and its P and dP columns.
+static size_t grad_bytes_per_instance(const DevPlan &pl, int dtype) {
+    const size_t hb = elt_hbytes(dtype), pb = elt_pbytes(dtype);
+    const PlanLayout &lay = pl.lay;
+    return (size_t)(lay.nstates + 2 * lay.max_level) * hb + (size_t)std::max<int64_t>(pl.nseg, 1) * hb +
+           (size_t)lay.npairs * (pb + hb);
+}
+
+template <int DT, typename W>
+static void launch_rev_g(const DevCaps &caps, const Tune &tune, const DevPlan &pl, int L,
+                         const void *P, int64_t ldp, const void *Gs, void *Gc, int64_t nb,
+                         hipStream_t st) {
+    using E = Elt<DT>;
+    const PlanLayout &lay = pl.lay;
+    int qlo, nq;
+    level_pairs(lay, L, &qlo, &nq);
+    const int64_t nch = lay.nst[L - 2];
+    const size_t lds = (size_t)nq * 64 * sizeof(typename E::P);
+    const bool use_lds = tune.lds && lds <= caps.max_lds_block;
+    const W *cent = (const W *)pl.d_cent;
+    const int64_t *coff = (const int64_t *)pl.d_coff + (lay.levels[L - 2] - 1);
+    if (use_lds) {
+        auto kern = k_rev_g<DT, W, kLdsWaves, true>;
+        static bool attr_set = false;
+        if (!attr_set) {
+            attr_set = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)caps.max_lds_block) == hipSuccess;
+            (void)hipGetLastError();
+        }
+        if (attr_set) {
+            const LdsMap map = make_lds_map(caps, tune.occ, nch, nb, lds);
+            kern<<<dim3((unsigned)(8 * map.per_xcd)), dim3(kLdsWaves * 64), lds, st>>>(
+                cent, coff, nch, qlo, nq, (const typename E::P *)P, ldp,
+                (const typename E::H *)Gs, nb, (typename E::H *)Gc, nb, map);
+            return;
+        }
+    }
+    const LdsMap map = make_lds_map(caps, 2, nch, nb, 0);
+    k_rev_g<DT, W, kLdsWaves, false><<<dim3((unsigned)(8 * map.per_xcd)), dim3(kLdsWaves * 64), 0, st>>>(
+        cent, coff, nch, qlo, nq, (const typename E::P *)P, ldp, (const typename E::H *)Gs, nb,
+        (typename E::H *)Gc, nb, map);
+}
+
+// Forward walk keeping every level, then the reverse sweep: out[b] = haf, dP[q][b] = d haf/d P.
+template <int DT, typename W>
+static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t ldp, int64_t B,
+                               void *out, void *dP, int64_t lddp, hipStream_t st,
+                               const Tune &tune) {
+    using E = Elt<DT>;
+    using HT = typename E::H;
+    const PlanLayout &lay = pl.lay;
+    EGG_TRY(ensure_rev(pl));
+    const size_t hb = sizeof(HT), pb = sizeof(typename E::P);
+    const int64_t chunk = tune.chunk > 0 ? std::min(tune.chunk, B) : B;
+    EGG_TRY(ensure(ds.Hg, (size_t)lay.nstates * chunk * hb));
+    EGG_TRY(ensure(ds.G, (size_t)2 * lay.max_level * chunk * hb));
+    EGG_TRY(ensure(ds.part, (size_t)std::max<int64_t>(pl.nseg, 1) * chunk * hb));
+    HT *H = (HT *)ds.Hg.p;
+    HT *G[2] = {(HT *)ds.G.p, (HT *)ds.G.p + (size_t)lay.max_level * chunk};
+    HT *part = (HT *)ds.part.p;
+    const int top = lay.K / 2 + 1;
+    for (int64_t b0 = 0; b0 < B; b0 += chunk) {
+        const int64_t nb = std::min(chunk, B - b0);
+        const char *Pc = (const char *)P + (size_t)b0 * pb;
+        HT *outc = (HT *)out + b0;
+        HT *dPc = (HT *)dP + b0;
+        auto level_rows = [&](int L) { return H + (size_t)(lay.levels[L - 1] - 1) * nb; };
+        k_fill_one<DT><<<dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st>>>(level_rows(1), nb);
+        for (int L = 2; L <= top; L++) {
+            const void *Hc = level_rows(L - 1);
+            void *Ho = L == top ? (void *)outc : (void *)level_rows(L);
+            if (!(tune.lds && launch_level_lds<DT, W>(ds.caps, tune, pl, L, Pc, ldp, Hc, nb, Ho, nb, nb, st)))
+                launch_level_bt<DT, W>(64, pl, L, Pc, ldp, Hc, nb, Ho, nb, nb, st);
+        }
+        // reverse: G of the full set is one
+        k_fill_one<DT><<<dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st>>>(G[top & 1], nb);
+        const int64_t ntiles = (nb + 63) / 64;
+        for (int L = top; L >= 2; L--) {
+            const int64_t s0 = pl.lvl_seg[L - 1], ns = pl.lvl_seg[L] - s0;
+            if (ns > 0) {
+                const int64_t tasks = ns * ntiles;
+                const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (tasks + 3) / 4));
+                k_rev_dp_seg<DT><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
+                    (const uint64_t *)pl.d_pent, (const RevSegDev *)pl.d_segs, s0, ns,
+                    level_rows(L - 1), nb, G[L & 1], nb, part, nb, ntiles);
+            }
+            if (L > 2)
+                launch_rev_g<DT, W>(ds.caps, tune, pl, L, Pc, ldp, G[L & 1], G[(L - 1) & 1], nb, st);
+        }
+        const int64_t total = lay.npairs * nb;
+        const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (total + 255) / 256));
+        k_rev_dp_reduce<DT><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
+            (const int64_t *)pl.d_qoff, (const int32_t *)pl.d_qseg, part, dPc, lddp, lay.npairs, nb);
+    }
+    EGG_HIP(hipGetLastError());
+    return EGGMAN_OK;
+}
+
+template <int DT>
+static int forward_grad_dt(DevState &ds, DevPlan &pl, const void *P, int64_t ldp, int64_t B,
+                           void *out, void *dP, int64_t lddp, hipStream_t st, const Tune &tune) {
+    if (pl.lay.index_bytes == 4)
+        return forward_grad_levels<DT, int32_t>(ds, pl, P, ldp, B, out, dP, lddp, st, tune);
+    return forward_grad_levels<DT, int64_t>(ds, pl, P, ldp, B, out, dP, lddp, st, tune);
+}
+
+static int forward_grad_any(int dtype, DevState &ds, DevPlan &pl, const void *P, int64_t ldp,
+                            int64_t B, void *out, void *dP, int64_t lddp, hipStream_t st,
+                            const Tune &tune) {
+    switch (dtype) {
+    case EGGMAN_F64: return forward_grad_dt<EGGMAN_F64>(ds, pl, P, ldp, B, out, dP, lddp, st, tune);
+    case EGGMAN_F32: return forward_grad_dt<EGGMAN_F32>(ds, pl, P, ldp, B, out, dP, lddp, st, tune);
+    case EGGMAN_C64: return forward_grad_dt<EGGMAN_C64>(ds, pl, P, ldp, B, out, dP, lddp, st, tune);
+    case EGGMAN_C32: return forward_grad_dt<EGGMAN_C32>(ds, pl, P, ldp, B, out, dP, lddp, st, tune);
+    case EGGMAN_I128: return forward_grad_dt<EGGMAN_I128>(ds, pl, P, ldp, B, out, dP, lddp, st, tune);
+    }
+    return fail(EGGMAN_EINVAL, "unknown dtype %d", dtype);
+}
+
+// Largest chunk of instances whose gradient buffers fit in `frac` of the free device memory.
+static int grad_chunk(const DevPlan &pl, int dtype, int64_t want, int64_t *chunk) {
+    size_t free_b = 0, total_b = 0;
+    EGG_HIP(hipMemGetInfo(&free_b, &total_b));
+    const size_t per = grad_bytes_per_instance(pl, dtype);
+    int64_t fit = (int64_t)((double)free_b * 0.8 / (double)per);
+    if (fit >= 64) fit -= fit % 64;
+    if (fit < 1)
+        return fail(EGGMAN_ENOMEM, "one instance of the degree-%d gradient needs %.2f GB; %.2f GB free",
+                    pl.lay.K, per / 1e9, free_b / 1e9);
+    *chunk = std::min(want, fit);
+    return EGGMAN_OK;
 }
 
 // ------------------------------------------------------------------------------ plans
@@ -431,8 +962,7 @@ int eggman_dp_forward(uint64_t plan, int dtype, const void *P_host, int64_t B, i
     const int64_t chunk = max_batch > 0 ? std::min(max_batch, B) : B;
     EGG_TRY(ensure(ds->P, (size_t)chunk * npairs * pb));
     EGG_TRY(ensure(ds->out, (size_t)chunk * hb));
-    Tune tune = default_tune(pl->lay, dtype, chunk);
-    tune.chunk = chunk;
+    const Tune tune = pick_tune(*pl, dtype, chunk);
     std::vector<char> stage;
     for (int64_t b0 = 0; b0 < B; b0 += chunk) {
         const int64_t nb = std::min(chunk, B - b0);
@@ -471,15 +1001,271 @@ int eggman_dp_forward_device(uint64_t plan, int dtype, const void *P_dev, int64_
     EGG_TRY(dev_state(pl->device, &ds));
     hipStream_t st = stream ? (hipStream_t)stream : ds->stream;
     EGG_TRY(chain_stream(*ds, st));
-    Tune tune = default_tune(pl->lay, dtype, B);
+    const Tune tune = pick_tune(*pl, dtype, B);
     return forward_any(dtype, *ds, *pl, P_dev, ldp, B, out_dev, st, tune);
+}
+
+// Time every launch choice for (plan, dtype, B) on synthetic pair values and keep the fastest.
+// All choices give bit-identical results, so tuning never changes an answer.
+int eggman_autotune(uint64_t plan, int dtype, int64_t B, int reps, char *report,
+                    int64_t report_len) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    DevPlan *pl;
+    EGG_TRY(find_plan(plan, &pl));
+    const int hb = elt_hbytes(dtype), pb = elt_pbytes(dtype);
+    if (!hb) return fail(EGGMAN_EINVAL, "unknown dtype %d", dtype);
+    if (B < 1) return fail(EGGMAN_EINVAL, "batch must be positive");
+    if (reps < 1) reps = 3;
+    DeviceGuard g(pl->device);
+    EGG_TRY(g.rc);
+    DevState *ds;
+    EGG_TRY(dev_state(pl->device, &ds));
+    EGG_TRY(chain_stream(*ds, ds->stream));
+    std::vector<Tune> cands;
+    if (B >= 64) {
+        std::vector<int64_t> chunks;
+        for (int64_t d : {1, 2, 4, 8}) {
+            int64_t c = (B / d + 63) / 64 * 64;
+            if (c >= 64 && std::find(chunks.begin(), chunks.end(), c) == chunks.end()) chunks.push_back(c);
+        }
+        for (int64_t c : chunks) {
+            for (int occ : {1, 2, 4}) cands.push_back(Tune{64, c, true, occ});
+            cands.push_back(Tune{64, c, false, 0});
+        }
+    } else {
+        const int bt = pow2_at_least(B);
+        cands.push_back(Tune{bt, B, false, 0});
+        if (bt != 64) cands.push_back(Tune{64, B, false, 0});
+        cands.push_back(Tune{64, B, true, 0});
+    }
+    DevBuf Pd, Od;
+    EGG_TRY(ensure(Pd, (size_t)B * pl->lay.npairs * pb));
+    EGG_TRY(ensure(Od, (size_t)B * hb));
+    // pair values: the bit pattern 0x3f in every byte, a moderate real or integer value
+    EGG_HIP(hipMemsetAsync(Pd.p, 0x3f, Pd.bytes, ds->stream));
+    hipEvent_t e0, e1;
+    EGG_HIP(hipEventCreate(&e0));
+    EGG_HIP(hipEventCreate(&e1));
+    std::string rep;
+    float best = 1e30f;
+    Tune best_t = cands[0];
+    int rc = EGGMAN_OK;
+    for (size_t ci = 0; ci < cands.size(); ci++) {
+        const Tune t = cands[ci];
+        // warm-up (sizes the buffers; a graph candidate captures here and replays below)
+        rc = forward_any(dtype, *ds, *pl, Pd.p, B, B, Od.p, ds->stream, t);
+        if (rc == EGGMAN_OK && t.graph) rc = forward_any(dtype, *ds, *pl, Pd.p, B, B, Od.p, ds->stream, t);
+        if (rc != EGGMAN_OK) break;
+        if (hipEventRecord(e0, ds->stream) != hipSuccess) { rc = fail(EGGMAN_EHIP, "event record"); break; }
+        for (int r = 0; r < reps && rc == EGGMAN_OK; r++)
+            rc = forward_any(dtype, *ds, *pl, Pd.p, B, B, Od.p, ds->stream, t);
+        if (rc != EGGMAN_OK) break;
+        float ms = 0;
+        if (hipEventRecord(e1, ds->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = fail(EGGMAN_EHIP, "event timing failed");
+            break;
+        }
+        ms /= reps;
+        char line[160];
+        snprintf(line, sizeof line, "bt=%d chunk=%lld lds=%d occ=%d graph=%d ms=%.4f\n", t.bt,
+                 (long long)t.chunk, (int)t.lds, t.occ, (int)t.graph, ms);
+        rep += line;
+        if (ms < best) {
+            best = ms;
+            best_t = t;
+        }
+        // after the eager candidates, try the best of them replayed from a graph
+        if (ci + 1 == cands.size() && !best_t.graph && !t.graph) {
+            Tune gt = best_t;
+            gt.graph = true;
+            cands.push_back(gt);
+        }
+    }
+    (void)hipStreamSynchronize(ds->stream);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(Pd.p);
+    (void)hipFree(Od.p);
+    if (rc != EGGMAN_OK) return rc;
+    g_tune[TuneKey(pl->device, pl->lay.K, pl->lay.index_bytes, dtype, batch_bucket(B))] = best_t;
+    char line[160];
+    snprintf(line, sizeof line, "best bt=%d chunk=%lld lds=%d occ=%d graph=%d ms=%.4f\n", best_t.bt,
+             (long long)best_t.chunk, (int)best_t.lds, best_t.occ, (int)best_t.graph, best);
+    rep += line;
+    if (report && report_len > 0) {
+        const size_t n = std::min<size_t>(rep.size(), (size_t)report_len - 1);
+        std::memcpy(report, rep.data(), n);
+        report[n] = 0;
+    }
+    return EGGMAN_OK;
+}
+
+int eggman_tune_set(int K, int index_bytes, int dtype, int64_t B, int bt, int64_t chunk, int lds,
+                    int occ, int graph) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    if (!elt_hbytes(dtype) || B < 1 || (bt & (bt - 1)) || bt < 1 || bt > 64 || occ < 0)
+        return fail(EGGMAN_EINVAL, "bad tuning entry");
+    int dev;
+    EGG_TRY(current_device(&dev));
+    g_tune[TuneKey(dev, K, index_bytes, dtype, batch_bucket(B))] =
+        Tune{bt, chunk, lds != 0, occ, graph != 0};
+    return EGGMAN_OK;
+}
+
+int eggman_tune_get(int K, int index_bytes, int dtype, int64_t B, int *bt, int64_t *chunk,
+                    int *lds, int *occ, int *graph) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    int dev;
+    EGG_TRY(current_device(&dev));
+    auto it = g_tune.find(TuneKey(dev, K, index_bytes, dtype, batch_bucket(B)));
+    if (it == g_tune.end()) return fail(EGGMAN_EINVAL, "no tuning entry");
+    if (bt) *bt = it->second.bt;
+    if (chunk) *chunk = it->second.chunk;
+    if (lds) *lds = it->second.lds;
+    if (occ) *occ = it->second.occ;
+    if (graph) *graph = it->second.graph;
+    return EGGMAN_OK;
+}
+
+int eggman_dp_forward_mixed(int ngroups, const uint64_t *plans, const int64_t *counts, int dtype,
+                            const void *P_host, void *out_host) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    std::vector<DevPlan *> pls;
+    std::vector<int64_t> cnt;
+    int64_t np_total, total;
+    EGG_TRY(mixed_plans(ngroups, plans, counts, &pls, &cnt, &np_total, &total));
+    const int hb = elt_hbytes(dtype), pb = elt_pbytes(dtype);
+    if (!hb) return fail(EGGMAN_EINVAL, "unknown dtype %d", dtype);
+    if (total == 0) return EGGMAN_OK;
+    if (!P_host || !out_host) return fail(EGGMAN_EINVAL, "null P or out pointer");
+    DeviceGuard g(pls[0]->device);
+    EGG_TRY(g.rc);
+    DevState *ds;
+    EGG_TRY(dev_state(pls[0]->device, &ds));
+    EGG_TRY(chain_stream(*ds, ds->stream));
+    EGG_TRY(ensure(ds->P, (size_t)np_total * pb));
+    EGG_TRY(ensure(ds->out, (size_t)total * hb));
+    EGG_HIP(hipMemcpyAsync(ds->P.p, P_host, (size_t)np_total * pb, hipMemcpyHostToDevice, ds->stream));
+    Tune tune;
+    EGG_TRY(forward_mixed_any(dtype, *ds, pls, cnt, ds->P.p, ds->out.p, ds->stream, tune));
+    EGG_HIP(hipMemcpyAsync(out_host, ds->out.p, (size_t)total * hb, hipMemcpyDeviceToHost, ds->stream));
+    EGG_HIP(hipStreamSynchronize(ds->stream));
+    return EGGMAN_OK;
+}
+
+int eggman_dp_forward_mixed_device(int ngroups, const uint64_t *plans, const int64_t *counts,
+                                   int dtype, const void *P_dev, void *out_dev, void *stream) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    std::vector<DevPlan *> pls;
+    std::vector<int64_t> cnt;
+    int64_t np_total, total;
+    EGG_TRY(mixed_plans(ngroups, plans, counts, &pls, &cnt, &np_total, &total));
+    if (!elt_hbytes(dtype)) return fail(EGGMAN_EINVAL, "unknown dtype %d", dtype);
+    if (total == 0) return EGGMAN_OK;
+    if (!P_dev || !out_dev) return fail(EGGMAN_EINVAL, "null P or out pointer");
+    DeviceGuard g(pls[0]->device);
+    EGG_TRY(g.rc);
+    DevState *ds;
+    EGG_TRY(dev_state(pls[0]->device, &ds));
+    hipStream_t st = stream ? (hipStream_t)stream : ds->stream;
+    EGG_TRY(chain_stream(*ds, st));
+    Tune tune;
+    return forward_mixed_any(dtype, *ds, pls, cnt, P_dev, out_dev, st, tune);
+}
+
+int eggman_dp_forward_grad(uint64_t plan, int dtype, const void *P_host, int64_t B,
+                           int64_t npairs, int64_t max_batch, void *out_host, void *dP_host) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    DevPlan *pl;
+    EGG_TRY(find_plan(plan, &pl));
+    const int hb = elt_hbytes(dtype), pb = elt_pbytes(dtype);
+    if (!hb) return fail(EGGMAN_EINVAL, "unknown dtype %d", dtype);
+    if (B < 0) return fail(EGGMAN_EINVAL, "negative batch size");
+    if (npairs != pl->lay.npairs)
+        return fail(EGGMAN_EINVAL, "P has %lld pair rows, the degree-%d plan expects %lld",
+                    (long long)npairs, pl->lay.K, (long long)pl->lay.npairs);
+    if (B == 0) return EGGMAN_OK;
+    if (!P_host || !out_host || !dP_host) return fail(EGGMAN_EINVAL, "null P, out or dP pointer");
+    DeviceGuard g(pl->device);
+    EGG_TRY(g.rc);
+    DevState *ds;
+    EGG_TRY(dev_state(pl->device, &ds));
+    EGG_TRY(chain_stream(*ds, ds->stream));
+    EGG_TRY(ensure_rev(*pl));
+    int64_t chunk = max_batch > 0 ? std::min(max_batch, B) : B;
+    EGG_TRY(grad_chunk(*pl, dtype, chunk, &chunk));
+    EGG_TRY(ensure(ds->P, (size_t)chunk * npairs * pb));
+    EGG_TRY(ensure(ds->dP, (size_t)chunk * npairs * hb));
+    EGG_TRY(ensure(ds->out, (size_t)chunk * hb));
+    Tune tune = default_tune(pl->lay, dtype, chunk);
+    tune.chunk = chunk;
+    std::vector<char> stage, gstage;
+    for (int64_t b0 = 0; b0 < B; b0 += chunk) {
+        const int64_t nb = std::min(chunk, B - b0);
+        const void *src = P_host;
+        if (nb != B) {
+            stage.resize((size_t)nb * npairs * pb);
+            for (int64_t q = 0; q < npairs; q++)
+                std::memcpy(stage.data() + (size_t)q * nb * pb,
+                            (const char *)P_host + ((size_t)q * B + b0) * pb, (size_t)nb * pb);
+            src = stage.data();
+        }
+        EGG_HIP(hipMemcpyAsync(ds->P.p, src, (size_t)nb * npairs * pb, hipMemcpyHostToDevice,
+                               ds->stream));
+        EGG_TRY(forward_grad_any(dtype, *ds, *pl, ds->P.p, nb, nb, ds->out.p, ds->dP.p, nb,
+                                 ds->stream, tune));
+        EGG_HIP(hipMemcpyAsync((char *)out_host + (size_t)b0 * hb, ds->out.p, (size_t)nb * hb,
+                               hipMemcpyDeviceToHost, ds->stream));
+        if (nb == B) {
+            EGG_HIP(hipMemcpyAsync(dP_host, ds->dP.p, (size_t)nb * npairs * hb,
+                                   hipMemcpyDeviceToHost, ds->stream));
+            EGG_HIP(hipStreamSynchronize(ds->stream));
+        } else {
+            gstage.resize((size_t)nb * npairs * hb);
+            EGG_HIP(hipMemcpyAsync(gstage.data(), ds->dP.p, gstage.size(), hipMemcpyDeviceToHost,
+                                   ds->stream));
+            EGG_HIP(hipStreamSynchronize(ds->stream));
+            for (int64_t q = 0; q < npairs; q++)
+                std::memcpy((char *)dP_host + ((size_t)q * B + b0) * hb,
+                            gstage.data() + (size_t)q * nb * hb, (size_t)nb * hb);
+        }
+    }
+    return EGGMAN_OK;
+}
+
+int eggman_dp_forward_grad_device(uint64_t plan, int dtype, const void *P_dev, int64_t ldp,
+                                  int64_t B, void *out_dev, void *dP_dev, int64_t lddp,
+                                  void *stream) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    DevPlan *pl;
+    EGG_TRY(find_plan(plan, &pl));
+    if (!elt_hbytes(dtype)) return fail(EGGMAN_EINVAL, "unknown dtype %d", dtype);
+    if (B < 0 || ldp < B || lddp < B)
+        return fail(EGGMAN_EINVAL, "need 0 <= B <= ldp, lddp (B=%lld, ldp=%lld, lddp=%lld)",
+                    (long long)B, (long long)ldp, (long long)lddp);
+    if (B == 0) return EGGMAN_OK;
+    if (!P_dev || !out_dev || !dP_dev) return fail(EGGMAN_EINVAL, "null P, out or dP pointer");
+    DeviceGuard g(pl->device);
+    EGG_TRY(g.rc);
+    DevState *ds;
+    EGG_TRY(dev_state(pl->device, &ds));
+    hipStream_t st = stream ? (hipStream_t)stream : ds->stream;
+    EGG_TRY(chain_stream(*ds, st));
+    EGG_TRY(ensure_rev(*pl));
+    int64_t chunk = B;
+    EGG_TRY(grad_chunk(*pl, dtype, B, &chunk));
+    Tune tune = default_tune(pl->lay, dtype, chunk);
+    tune.chunk = chunk;
+    return forward_grad_any(dtype, *ds, *pl, P_dev, ldp, B, out_dev, dP_dev, lddp, st, tune);
 }
 
 int64_t eggman_cache_bytes(void) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     int64_t total = 0;
     for (auto &kv : g_plans) total += (int64_t)kv.second->bytes;
-    for (auto &kv : g_dev) total += (int64_t)(kv.second.H.bytes + kv.second.P.bytes + kv.second.out.bytes);
+    for (auto &kv : g_plans) total += (int64_t)kv.second->rev_bytes;
+    for (auto &kv : g_dev) total += (int64_t)kv.second.bytes();
     return total;
 }
 
@@ -489,7 +1275,10 @@ int eggman_cache_clear(void) {
     for (auto &kv : g_dev) {
         DeviceGuard g(kv.first);
         if (hipDeviceSynchronize() != hipSuccess) rc = fail(EGGMAN_EHIP, "synchronize failed");
-        for (DevBuf *b : {&kv.second.H, &kv.second.P, &kv.second.out}) {
+        for (GraphEntry &ge : kv.second.graphs) (void)hipGraphExecDestroy(ge.exec);
+        kv.second.graphs.clear();
+        for (DevBuf *b : {&kv.second.H, &kv.second.P, &kv.second.out, &kv.second.Hg,
+                          &kv.second.G, &kv.second.part, &kv.second.dP}) {
             if (b->p) (void)hipFree(b->p);
             b->p = nullptr;
             b->bytes = 0;
@@ -497,7 +1286,9 @@ int eggman_cache_clear(void) {
     }
     for (auto &kv : g_plans) {
         DeviceGuard g(kv.second->device);
-        if (kv.second->d_trans) (void)hipFree(kv.second->d_trans);
+        DevPlan &p = *kv.second;
+        for (void *ptr : {p.d_trans, p.d_cent, p.d_coff, p.d_pent, p.d_segs, p.d_qoff, p.d_qseg})
+            if (ptr) (void)hipFree(ptr);
     }
     g_plans.clear();
     g_plan_key.clear();

@@ -72,6 +72,131 @@ __global__ __launch_bounds__(kBlock) void k_level_fwd(
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Batched level kernel with the level's pair values staged in LDS and an XCD-aware mapping.
+//
+// Every state of level L has min(S) = p in [p_lo, p_hi] (plan_rank.hpp), so its transitions only
+// read the pair rows p_lo..p_hi, which are one contiguous range [qlo, qhi] of pair indices --
+// 105..140 of the 378 pairs at K = 28.  A workgroup owns one 64-instance tile, copies that range
+// of P for its tile into LDS once, and then walks many states of the level with one wave per
+// state: the only global reads left per transition are the wave-uniform plan word (scalar) and
+// the 64 consecutive child values.
+//
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup j runs on XCD j % 8).  The mapping
+// below gives each XCD whole tiles when there are at least 8, and a contiguous share of the
+// states of a tile otherwise, so a tile's child rows are only ever cached in one XCD's L2.  The
+// workgroups of one XCD sweep the states of a tile in plan order, which is colexicographic in
+// the subset, and consecutive states share most of their children -- that sweep is what lets
+// the L2 serve most child reads.
+struct LdsMap {
+    int64_t ntiles;      // 64-instance tiles in the chunk
+    int64_t per_xcd;     // workgroups per XCD (R)
+    int64_t tiles_xcd;   // tiles per XCD when ntiles >= 8 (ceil(ntiles / 8)), else 0
+};
+
+// Arguments of one level of one group of instances sharing a plan.
+template <int DT, typename W>
+struct LevelArgs {
+    const W *trans_lvl;                       // the level's first transition
+    int64_t nst;                              // states in the level
+    int k;                                    // transitions per state
+    int qlo, nq;                              // pair rows [qlo, qlo + nq) read by the level
+    const typename Elt<DT>::P *P;             // pair values, rows ldp apart
+    int64_t ldp;
+    const typename Elt<DT>::H *Hc;            // the level below; row 0 is state cfirst
+    int64_t cfirst, ldh;
+    typename Elt<DT>::H *Ho;                  // this level, rows ldo apart
+    int64_t ldo, nb;                          // instances in the group
+    LdsMap map;
+};
+
+// The body of the LDS kernel for workgroup `j` of a launch that gives this level
+// 8 * map.per_xcd workgroups (j counts from a multiple of 8, so j % 8 is still the XCD).
+template <int DT, typename W, int WPB>
+__device__ inline void level_lds_body(const LevelArgs<DT, W> &a, int64_t j,
+                                      typename Elt<DT>::P *lds) {
+    using E = Elt<DT>;
+    using PT = typename E::P;
+    constexpr int bits = Packing<W>::bits;
+    constexpr W mask = (W)((W(1) << bits) - 1);
+    const LdsMap &map = a.map;
+    const int lane = (int)(threadIdx.x & 63);
+    const int wave = (int)(threadIdx.x >> 6);
+    const int64_t xcd = j & 7, r = j >> 3;
+    int64_t tile, s_begin, s_end, bi, bstride;
+    if (map.tiles_xcd > 0) {
+        const int64_t ti = r % map.tiles_xcd;
+        tile = xcd + 8 * ti;
+        bi = r / map.tiles_xcd;
+        bstride = map.per_xcd / map.tiles_xcd;
+        s_begin = 0;
+        s_end = a.nst;
+        if (tile >= map.ntiles) return;
+    } else {
+        tile = xcd % map.ntiles;
+        const int64_t group = xcd / map.ntiles;
+        const int64_t ngroups = (8 - tile + map.ntiles - 1) / map.ntiles;
+        s_begin = group * a.nst / ngroups;
+        s_end = (group + 1) * a.nst / ngroups;
+        bi = r;
+        bstride = map.per_xcd;
+    }
+    const int64_t b = tile * 64 + lane;
+    const bool live = b < a.nb;
+
+    // stage pair rows [qlo, qlo + nq) of this tile
+    for (int i = wave; i < a.nq; i += WPB)
+        lds[i * 64 + lane] = live ? a.P[(int64_t)(a.qlo + i) * a.ldp + b] : PT{};
+    __syncthreads();
+
+    const int64_t step = bstride * WPB;
+    for (int64_t s0 = s_begin + bi * WPB; s0 < s_end; s0 += step) {
+        const int64_t s = wave_uniform(s0 + wave);
+        if (s >= s_end) break;
+        const W *tw = a.trans_lvl + s * (int64_t)a.k;
+        typename E::H acc = E::zero();
+        if (live) {
+#pragma unroll 4
+            for (int i = 0; i < a.k; i++) {
+                const W t = tw[i];
+                const int q = (int)(t & mask) - 1 - a.qlo;
+                const int64_t c = (int64_t)(t >> bits) - a.cfirst;
+                acc = E::muladd(acc, lds[q * 64 + lane], a.Hc[c * a.ldh + b]);
+            }
+            a.Ho[s * a.ldo + b] = acc;
+        }
+    }
+}
+
+template <int DT, typename W, int WPB>
+__global__ __launch_bounds__(WPB * 64) void k_level_fwd_lds(LevelArgs<DT, W> a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    level_lds_body<DT, W, WPB>(a, (int64_t)blockIdx.x,
+                               reinterpret_cast<typename Elt<DT>::P *>(lds_raw));
+}
+
+// Several groups of instances, each with its own plan (degree), in one launch: group g owns
+// workgroups [first[g], first[g+1]) of the grid, each range starting at a multiple of 8.  This is
+// how a batch of mixed degrees advances one level of every degree per launch.
+constexpr int kMaxGroups = 8;
+
+template <int DT, typename W>
+struct MultiLevelArgs {
+    int n;
+    int64_t first[kMaxGroups + 1];
+    LevelArgs<DT, W> g[kMaxGroups];
+};
+
+template <int DT, typename W, int WPB>
+__global__ __launch_bounds__(WPB * 64) void k_level_fwd_lds_multi(MultiLevelArgs<DT, W> m) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int64_t j = blockIdx.x;
+    int g = 0;
+    while (g + 1 < m.n && j >= m.first[g + 1]) g++;
+    level_lds_body<DT, W, WPB>(m.g[g], j - m.first[g],
+                               reinterpret_cast<typename Elt<DT>::P *>(lds_raw));
+}
+
 // Row of ones for the empty set (H[1] = one, src/dp.jl:421): a plain store kernel, so the
 // sequence stays valid inside a captured graph.
 template <int DT>

@@ -133,6 +133,72 @@ int build_plan_host(const PlanLayout &lay, void *starts, void *trans, void *leve
                                  nthreads);
 }
 
+template <typename W>
+static int build_rev_t(const PlanLayout &lay, const W *trans, int seg_len, RevPlanHost *rp) {
+    const int64_t mask = ((int64_t)1 << lay.bits) - 1;
+    rp->coff.assign(lay.nstates + 1, 0);
+    rp->cent.assign((size_t)lay.ntrans * sizeof(W), 0);
+    rp->pent.assign(lay.ntrans, 0);
+    rp->segs.clear();
+    rp->lvl_seg.assign(lay.nlevels, 0);
+    W *cent = (W *)rp->cent.data();
+    // parents per child (each child belongs to the level just below its parents)
+    for (int64_t t = 0; t < lay.ntrans; t++) rp->coff[((int64_t)trans[t] >> lay.bits) - 1 + 1]++;
+    for (int64_t s = 0; s < lay.nstates; s++) rp->coff[s + 1] += rp->coff[s];
+    std::vector<int64_t> fill(rp->coff.begin(), rp->coff.end() - 1);
+    std::vector<std::vector<int32_t>> per_pair(lay.npairs);
+    for (int L = 2; L <= lay.K / 2 + 1; L++) {
+        const int k = lay.size[L - 1] - 1;
+        const int64_t cfirst = lay.levels[L - 2];
+        const int64_t t0 = lay.tstart[L - 1], t1 = lay.tstart[L];
+        // child-major, parents visited in ascending order
+        for (int64_t t = t0; t < t1; t++) {
+            const int64_t w = (int64_t)trans[t];
+            const int64_t srank = (t - t0) / k;
+            const int64_t c = (w >> lay.bits) - 1;
+            cent[fill[c]++] = (W)((srank << lay.bits) | (w & mask));
+        }
+        // pair-major: counting sort of the level's transitions by pair, stable in parent order
+        std::vector<int64_t> cnt(lay.npairs + 1, 0);
+        for (int64_t t = t0; t < t1; t++) cnt[((int64_t)trans[t] & mask) - 1 + 1]++;
+        for (int64_t q = 0; q < lay.npairs; q++) cnt[q + 1] += cnt[q];
+        std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+        for (int64_t t = t0; t < t1; t++) {
+            const int64_t w = (int64_t)trans[t];
+            const int64_t q = (w & mask) - 1;
+            const uint64_t srank = (uint64_t)((t - t0) / k);
+            const uint64_t crank = (uint64_t)((w >> lay.bits) - cfirst);
+            rp->pent[t0 + pos[q]++] = (srank << 32) | crank;
+        }
+        rp->lvl_seg[L - 1] = (int64_t)rp->segs.size();
+        for (int64_t q = 0; q < lay.npairs; q++) {
+            for (int64_t a = cnt[q]; a < cnt[q + 1]; a += seg_len) {
+                RevSeg sg;
+                sg.start = t0 + a;
+                sg.len = (int32_t)std::min<int64_t>(seg_len, cnt[q + 1] - a);
+                sg.level = L;
+                per_pair[q].push_back((int32_t)rp->segs.size());
+                rp->segs.push_back(sg);
+            }
+        }
+    }
+    rp->lvl_seg[lay.nlevels - 1] = (int64_t)rp->segs.size();
+    rp->qoff.assign(lay.npairs + 1, 0);
+    rp->qseg.clear();
+    for (int64_t q = 0; q < lay.npairs; q++) {
+        rp->qoff[q] = (int64_t)rp->qseg.size();
+        rp->qseg.insert(rp->qseg.end(), per_pair[q].begin(), per_pair[q].end());
+    }
+    rp->qoff[lay.npairs] = (int64_t)rp->qseg.size();
+    return EGGMAN_OK;
+}
+
+int build_rev_plan(const PlanLayout &lay, const void *trans, int seg_len, RevPlanHost *rp) {
+    if (seg_len < 1) return fail(EGGMAN_EINVAL, "segment length must be positive");
+    if (lay.index_bytes == 4) return build_rev_t<int32_t>(lay, (const int32_t *)trans, seg_len, rp);
+    return build_rev_t<int64_t>(lay, (const int64_t *)trans, seg_len, rp);
+}
+
 // Check that caller-supplied arrays are a plan the kernels can walk safely: the closed-form
 // sizes and level boundaries, a uniform out-degree per level (what lets the device drop
 // `starts`), and every transition pointing into the level below with a valid pair index.
@@ -159,10 +225,14 @@ static int validate_t(const PlanLayout &lay, const W *starts, const W *trans, co
     std::atomic<int> bad(0);
     auto check = [&](int L, int64_t t0, int64_t t1) {
         const int64_t cf = lay.levels[L - 2], cl = lay.levels[L - 1];   // child level [cf, cl)
+        // a state of this level has min(S) in [lo, K - m + 1], so its pairs lie in these rows
+        const int m = lay.size[L - 1];
+        const int64_t plo = lay.lo[L - 1], phi = std::min(lay.K - m + 1, lay.K - 1);
+        const int64_t qlo = pair_index(plo, plo + 1, lay.K), qhi = pair_index(phi, lay.K, lay.K);
         for (int64_t t = t0; t < t1; t++) {
             const int64_t w = (int64_t)trans[t];
             const int64_t q = w & mask, c = w >> lay.bits;
-            if (q < 1 || q > lay.npairs || c < cf || c >= cl) { bad.store(1); return; }
+            if (q < qlo || q > qhi || c < cf || c >= cl) { bad.store(1); return; }
         }
     };
     if (nthreads <= 0) nthreads = (int)std::max(1u, std::thread::hardware_concurrency());

@@ -7,15 +7,21 @@ package loads the library; there is no CPU fallback.
 """
 from ._lib import (ArgumentError, DimensionMismatch, EggmanError, F32, F64, C32, C64, I128,
                    lib as _load_lib)
-from .plan import (DP_MAX, DP_HARD_MAX, UNROLL_MAX, HafnianDPPlan, _build_dp_plan, _dp_counts,
+from .plan import (DP_MAX, DP_HARD_MAX, UNROLL_MAX, HafnianDPPlan, PlanRef, _build_dp_plan, _dp_counts,
                    _dp_index_type, _dp_plan, _pair_bits, _pair_index, _pair_mask)
-from .backend import (HIPBackend, _device_plan, _haf_dp_backend, _haf_dp_device,
+from .backend import (HIPBackend, _device_plan, _haf_dp_backend, _haf_dp_backend_grad,
+                      _haf_dp_backend_mixed,
+                      _haf_dp_device,
                       dp_batch_bytes, empty_gpu_cache, gpu_cache_bytes)
 from .hafnian import (_gather_pairs_batch, default_backend, hafnian, hafnian_exact,
-                      hafnian_repeated, reduction)
+                      hafnian_grad, hafnian_mixed, hafnian_repeated, hafnian_repeated_grad,
+                      reduction, rrule)
+
+from .tuning import autotune, load_tuning, save_tuning, tune_get, tune_set
 
 _load_lib()
 
-__all__ = ["hafnian", "hafnian_repeated", "hafnian_exact", "reduction", "HIPBackend",
+__all__ = ["hafnian", "hafnian_repeated", "hafnian_exact", "hafnian_grad", "hafnian_mixed",
+           "hafnian_repeated_grad", "rrule", "reduction", "HIPBackend",
            "gpu_cache_bytes", "empty_gpu_cache", "dp_batch_bytes", "DP_MAX", "DP_HARD_MAX",
            "HafnianDPPlan", "ArgumentError", "DimensionMismatch", "EggmanError"]

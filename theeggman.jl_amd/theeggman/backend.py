@@ -114,6 +114,83 @@ def _haf_dp_backend(backend: HIPBackend, P: np.ndarray, plan: HafnianDPPlan, *,
     return out
 
 
+def _haf_dp_backend_grad(backend: HIPBackend, P: np.ndarray, plan: HafnianDPPlan, *,
+                         max_batch: int | None = None):
+    """Hafnians and d haf / d P of the B instances in ``P`` (shape (B, npairs)).
+
+    Returns ``(values, dP)`` with dP of shape (B, npairs) and P's element type (for int64 P:
+    lists of Python ints).  New entry point: the reference has no derivative.
+    """
+    if not isinstance(backend, HIPBackend):
+        raise ArgumentError(f"no GPU backend method for {type(backend).__name__}")
+    P = np.asarray(P)
+    if P.ndim != 2:
+        raise DimensionMismatch(f"P must be B x npairs, got shape {P.shape}")
+    B, npairs = P.shape
+    if npairs != plan.K * (plan.K - 1) // 2:
+        raise DimensionMismatch(f"P has {npairs} pair columns, the degree-{plan.K} plan "
+                                f"expects {plan.K * (plan.K - 1) // 2}")
+    code = dtype_code(P.dtype)
+    if B == 0:
+        return (np.empty(0, P.dtype), np.empty((0, npairs), P.dtype)) if code != I128 else ([], [])
+    h = _device_plan(backend, plan)
+    Pf = _as_pair_major(P)
+    if code == I128:
+        out = np.empty((B, 2), np.uint64)
+        dP = np.empty((npairs, B, 2), np.uint64)
+    else:
+        out = np.empty(B, P.dtype)
+        dP = np.empty((npairs, B), P.dtype)
+    mb = 0 if max_batch is None else max(1, min(B, int(max_batch)))
+    check(lib().eggman_dp_forward_grad(h, code, Pf.ctypes.data, B, npairs, mb, out.ctypes.data,
+                                       dP.ctypes.data), "dp_forward_grad")
+    if code == I128:
+        vals = [_limbs_to_int(int(lo), int(hi)) for lo, hi in out]
+        grads = [[_limbs_to_int(int(dP[q, b, 0]), int(dP[q, b, 1])) for q in range(npairs)]
+                 for b in range(B)]
+        return vals, grads
+    return out, dP.T
+
+
+def _haf_dp_backend_mixed(backend: HIPBackend, Ps, plans):
+    """Hafnians of several groups of instances, group g of degree plans[g].K with pair values
+    Ps[g] (shape (B_g, npairs_g)), in one device pass that advances every group by one level per
+    launch.  Returns one array (or list of ints, exact path) per group.  New entry point."""
+    if not isinstance(backend, HIPBackend):
+        raise ArgumentError(f"no GPU backend method for {type(backend).__name__}")
+    if len(Ps) != len(plans) or not Ps:
+        raise DimensionMismatch("need one P block per plan")
+    dt = np.result_type(*[np.asarray(P).dtype for P in Ps])
+    code = dtype_code(dt)
+    blocks, counts, handles = [], [], []
+    for P, plan in zip(Ps, plans):
+        P = np.asarray(P, dtype=dt)
+        B, npairs = P.shape
+        if npairs != plan.K * (plan.K - 1) // 2:
+            raise DimensionMismatch(f"P block has {npairs} pair columns, its degree-{plan.K} plan "
+                                    f"expects {plan.K * (plan.K - 1) // 2}")
+        blocks.append(np.ascontiguousarray(P.T).ravel())      # [pair][batch]
+        counts.append(B)
+        handles.append(_device_plan(backend, plan))
+    total = int(sum(counts))
+    if total == 0:
+        return [np.empty(0, dt) for _ in Ps]
+    flat = np.concatenate(blocks)
+    out = np.empty((total, 2), np.uint64) if code == I128 else np.empty(total, dt)
+    h = (ctypes.c_uint64 * len(handles))(*handles)
+    c = (ctypes.c_int64 * len(counts))(*counts)
+    check(lib().eggman_dp_forward_mixed(len(handles), h, c, code, flat.ctypes.data,
+                                        out.ctypes.data), "dp_forward_mixed")
+    res, o = [], 0
+    for B in counts:
+        if code == I128:
+            res.append([_limbs_to_int(int(lo), int(hi)) for lo, hi in out[o:o + B]])
+        else:
+            res.append(out[o:o + B])
+        o += B
+    return res
+
+
 def _limbs_to_int(lo: int, hi: int) -> int:
     v = (hi << 64) | lo
     return v - (1 << 128) if v >> 127 else v

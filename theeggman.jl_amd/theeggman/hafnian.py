@@ -11,8 +11,9 @@ from __future__ import annotations
 import numpy as np
 
 from ._lib import ArgumentError, DimensionMismatch, I128
-from .backend import HIPBackend, _haf_dp_backend, dtype_code
-from .plan import DP_HARD_MAX, DP_MAX, UNROLL_MAX, _dp_plan
+from .backend import (HIPBackend, _haf_dp_backend, _haf_dp_backend_grad,
+                      _haf_dp_backend_mixed, dtype_code)
+from .plan import DP_HARD_MAX, DP_MAX, UNROLL_MAX, PlanRef, _dp_plan
 
 _DEFAULT_BACKEND = None
 
@@ -91,9 +92,13 @@ def _route(method: str, K: int) -> None:
                                   "Int64-layout DP on the device")
 
 
-def _plan_for(K: int):
-    """_dp_plan(K, Int32) through DP_MAX, Int64 above (src/hafnian.jl:1044-1049)."""
-    return _dp_plan(K, np.int32 if K <= DP_MAX else np.int64)
+def _plan_for(K: int, backend=None):
+    """_dp_plan(K, Int32) through DP_MAX, Int64 above (src/hafnian.jl:1044-1049).  A backend that
+    generates plans on the device only needs the degree and layout, so no host plan is built."""
+    I = np.int32 if K <= DP_MAX else np.int64
+    if isinstance(backend, HIPBackend) and backend.plan_source == "generate":
+        return PlanRef(K, np.dtype(I).itemsize)
+    return _dp_plan(K, I)
 
 
 _TRIU_CACHE: dict = {}
@@ -184,7 +189,7 @@ def hafnian(A, *, method: str = "auto", check_symmetric: bool = True, backend=No
         _check_symmetric(A)
     _route(method, N)
     P = _gather_pairs_batch(T, A, None, N)
-    return _haf_dp_backend(_backend_or_default(backend), P, _plan_for(N))[0]
+    return _haf_dp_backend(_backend_or_default(backend), P, _plan_for(N, backend))[0]
 
 
 def _hafnian_batch(As, *, method, check_symmetric, backend, max_batch):
@@ -209,7 +214,7 @@ def _hafnian_batch(As, *, method, check_symmetric, backend, max_batch):
     _route(method, N)
     stack = np.asarray(As) if isinstance(As, list) else As
     P = _gather_pairs_batch(T, stack, None, N)
-    return _haf_dp_backend(_backend_or_default(backend), P, _plan_for(N), max_batch=max_batch)
+    return _haf_dp_backend(_backend_or_default(backend), P, _plan_for(N, backend), max_batch=max_batch)
 
 
 def hafnian_repeated(A, rpt, *, method: str = "auto", check_symmetric: bool = True,
@@ -240,7 +245,7 @@ def hafnian_repeated(A, rpt, *, method: str = "auto", check_symmetric: bool = Tr
     _route(method, total)
     idx = _expanded_indices(r, total)
     P = _gather_pairs_batch(T, A, [idx], total)
-    return _haf_dp_backend(_backend_or_default(backend), P, _plan_for(total))[0]
+    return _haf_dp_backend(_backend_or_default(backend), P, _plan_for(total, backend))[0]
 
 
 def _hafnian_repeated_batch(A, rpts, *, method, check_symmetric, backend, max_batch):
@@ -273,7 +278,7 @@ def _hafnian_repeated_batch(A, rpts, *, method, check_symmetric, backend, max_ba
     # (src/hafnian.jl:1348)
     idxs = [_expanded_indices(np.asarray(r, dtype=np.int64), total) for r in rpts]
     P = _gather_pairs_batch(T, A, idxs, total)
-    return _haf_dp_backend(_backend_or_default(backend), P, _plan_for(total),
+    return _haf_dp_backend(_backend_or_default(backend), P, _plan_for(total, backend),
                            max_batch=max_batch)
 
 
@@ -310,9 +315,136 @@ def hafnian_exact(A, *, check_symmetric: bool = True, backend=None,
         _check_method("dp", N)
         stack = np.stack([M.astype(np.int64) for M in As])
         P = _gather_pairs_batch(np.int64, stack, None, N)
-        vals = _haf_dp_backend(_backend_or_default(backend), P, _plan_for(N),
+        vals = _haf_dp_backend(_backend_or_default(backend), P, _plan_for(N, backend),
                                max_batch=max_batch)
     return vals if batch else vals[0]
+
+
+def hafnian_mixed(As, *, check_symmetric: bool = True, backend=None):
+    """Hafnians of matrices of different sizes, in input order, in one device pass.
+
+    New entry point: the reference's batch form requires one size (src/hafnian.jl:1277-1280).
+    Matrices are grouped by degree; every group shares its plan, and one launch advances all
+    groups by a level.  Each value is bit-identical to ``hafnian`` of that matrix alone.
+    Degrees above DP_MAX (Int64 layout) and up to it (Int32) run as two passes.
+    """
+    As = [_as_matrix(A) for A in As]
+    if not As:
+        return np.empty(0)
+    T = _haf_eltype(np.result_type(*[A.dtype for A in As]))
+    out = np.zeros(len(As), T)
+    groups: dict = {}
+    for i, A in enumerate(As):
+        N = _checksquare(A)
+        if check_symmetric:
+            _check_symmetric(A)
+        if N == 0:
+            out[i] = 1
+        elif N % 2 == 0:
+            _check_method("dp", N)
+            groups.setdefault(N, []).append(i)
+    be = _backend_or_default(backend)
+    for wide in (False, True):
+        Ns = sorted(N for N in groups if (N > DP_MAX) == wide)
+        if not Ns:
+            continue
+        Ps = [_gather_pairs_batch(T, np.stack([As[i] for i in groups[N]]), None, N) for N in Ns]
+        res = _haf_dp_backend_mixed(be, Ps, [_plan_for(N, backend) for N in Ns])
+        for N, r in zip(Ns, res):
+            out[groups[N]] = r
+    return out
+
+
+def _scatter_pair_grad(dp_row, idx, n: int, dtype) -> np.ndarray:
+    """Map d haf / d P (one instance) back to the matrix through the gather of _gather_pairs.
+
+    Returns the symmetric G with G[u, v] = G[v, u] = d haf / d a_uv, the derivative with respect
+    to the symmetric pair variable a_uv = A[u, v] = A[v, u] (u != v) or the diagonal entry a_uu:
+    the sum of dP over every gathered pair (i < j) with {idx[i], idx[j]} = {u, v}.  With
+    distinct indices each G[u, v] is the hafnian of A with rows and columns u, v removed."""
+    K = len(idx)
+    iu, ju = _triu(K)
+    r = np.asarray(idx, dtype=np.int64) - 1
+    u, v = r[iu], r[ju]
+    G = np.zeros((n, n), dtype)
+    np.add.at(G, (u, v), dp_row)
+    off = u != v
+    np.add.at(G, (v[off], u[off]), dp_row[off])
+    return G
+
+
+def hafnian_grad(A, *, check_symmetric: bool = True, backend=None, max_batch: int | None = None):
+    """Hafnian and its gradient, for one matrix or a batch (list or (B, N, N) array).
+
+    Returns ``(value, G)`` (batch: arrays of values and of N x N gradients).  ``G`` is symmetric
+    with G[i, j] = G[j, i] = d haf / d a_ij for the symmetric pair variable a_ij, which equals
+    the hafnian of A with rows and columns i, j removed; the diagonal is zero.  Computed by a
+    reverse sweep over the same plan on the device (new: the reference has no derivative).
+    """
+    batch = _is_batch(A)
+    As = [_as_matrix(M) for M in A] if batch else [_as_matrix(A)]
+    B = len(As)
+    if B == 0:
+        return np.empty(0), np.empty((0, 0, 0))
+    T = _haf_eltype(np.result_type(*[M.dtype for M in As]))
+    N = _checksquare(As[0])
+    for b, M in enumerate(As):
+        if M.shape != (N, N):
+            raise DimensionMismatch(f"matrix {b + 1} is {M.shape[0]}×{M.shape[-1]}, expected "
+                                    f"{N}×{N}; a batch shares one plan")
+        if check_symmetric:
+            _check_symmetric(M)
+    if N % 2 or N == 0:
+        vals = np.full(B, 1 if N == 0 else 0, T)
+        grads = np.zeros((B, N, N), T)
+    else:
+        _check_method("dp", N)
+        _route("dp", N)
+        P = _gather_pairs_batch(T, np.stack(As), None, N)
+        vals, dP = _haf_dp_backend_grad(_backend_or_default(backend), P, _plan_for(N, backend),
+                                        max_batch=max_batch)
+        idx = np.arange(1, N + 1)
+        grads = np.stack([_scatter_pair_grad(dP[b], idx, N, T) for b in range(B)])
+    return (vals, grads) if batch else (vals[0], grads[0])
+
+
+def hafnian_repeated_grad(A, rpt, *, check_symmetric: bool = True, backend=None):
+    """Hafnian of A with row/column i repeated rpt[i] times, and its gradient with respect to
+    the entries of A (symmetric pair variables, diagonal included: repeated rows read it)."""
+    A = _as_matrix(A)
+    N = _checksquare(A)
+    r = np.asarray(rpt, dtype=np.int64)
+    if len(r) != N:
+        raise DimensionMismatch(f"rpt has length {len(r)}, expected {N}")
+    if (r < 0).any():
+        raise ArgumentError("rpt must contain non-negative integers")
+    T = _haf_eltype(A.dtype)
+    total = int(r.sum())
+    if check_symmetric:
+        _check_symmetric(A)
+    if total == 0 or total % 2:
+        return T.type(1 if total == 0 else 0), np.zeros((N, N), T)
+    _check_method("dp", total)
+    idx = _expanded_indices(r, total)
+    P = _gather_pairs_batch(T, A, [idx], total)
+    vals, dP = _haf_dp_backend_grad(_backend_or_default(backend), P, _plan_for(total, backend))
+    return vals[0], _scatter_pair_grad(dP[0], idx, N, T)
+
+
+def rrule(f, A, *args, **kwargs):
+    """ChainRulesCore-style reverse rule for ``hafnian``: returns ``(y, pullback)`` where
+    ``pullback(dy)`` gives ``(None, dy * G)`` with G from :func:`hafnian_grad`."""
+    if f is not hafnian:
+        raise ArgumentError("rrule is defined for hafnian only")
+    y, G = hafnian_grad(A, *args, **kwargs)
+
+    def pullback(dy):
+        dy = np.asarray(dy)
+        if G.ndim == 3:
+            return None, dy[:, None, None] * G
+        return None, dy * G
+
+    return y, pullback
 
 
 def reduction(A, rpt) -> np.ndarray:
@@ -324,5 +456,6 @@ def reduction(A, rpt) -> np.ndarray:
     return A[np.ix_(idx, idx)]
 
 
-__all__ = ["hafnian", "hafnian_repeated", "hafnian_exact", "reduction", "default_backend",
+__all__ = ["hafnian", "hafnian_repeated", "hafnian_exact", "hafnian_grad",
+           "hafnian_repeated_grad", "hafnian_mixed", "rrule", "reduction", "default_backend",
            "dtype_code", "I128"]

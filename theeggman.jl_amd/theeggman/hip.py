@@ -124,5 +124,11 @@ class Event:
         return ms.value
 
 
+def copy_d2d(dst: DeviceBuffer, src: DeviceBuffer, nbytes: int) -> None:
+    if nbytes > min(dst.nbytes, src.nbytes):
+        raise ValueError("copy larger than a buffer")
+    _chk(runtime().hipMemcpy(dst.ptr, src.ptr, nbytes, D2D), "hipMemcpy D2D")
+
+
 def device_synchronize() -> None:
     _chk(runtime().hipDeviceSynchronize(), "hipDeviceSynchronize")

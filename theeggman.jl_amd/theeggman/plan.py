@@ -59,6 +59,18 @@ class HafnianDPPlan:
         return self.trans.dtype.itemsize
 
 
+@dataclass(frozen=True)
+class PlanRef:
+    """A degree and layout without host arrays: enough for a backend that generates the plan
+    in device memory (``HIPBackend(plan_source="generate")``)."""
+    K: int
+    index_bytes: int
+
+    @property
+    def nstates(self) -> int:
+        return _dp_counts(self.K)[0]
+
+
 def _build_dp_plan(K: int, I=None) -> HafnianDPPlan:
     """Build the degree-K plan in layout ``I`` (np.int32 / np.int64), src/dp.jl:223-246.
 
