@@ -103,6 +103,8 @@ def main():
     ap.add_argument("--N", type=int, default=N_DEFAULT)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path lines")
+    ap.add_argument("--no-grad", action="store_true", help="skip the forward+gradient line")
+    ap.add_argument("--no-tune", action="store_true", help="use the default launch choices")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="after the timed steps, run one device-to-device copy of this many MiB "
                          "(a kernel of known traffic, to scale the HBM counters of a profile)")
@@ -115,7 +117,7 @@ def main():
 
     import numpy as np
     import theeggman as te
-    from theeggman import hip, synth
+    from theeggman import _lib, hip, synth
 
     K, B = args.N, args.batch
     be = te.HIPBackend()
@@ -133,6 +135,10 @@ def main():
     def step():
         te._haf_dp_device(be, plan, np.float64, Pd.addr, B, B, out.addr, stream.handle)
 
+    tuned = None
+    if not args.no_tune:
+        log("autotune batched")
+        tuned = te.autotune(K, np.float64, B, reps=2)["choice"]
     log("batched forward")
     ms, wall_ms = time_device(hip, stream, step, args.steps, args.warmup)
     # spot-check the timed result against the oracle (bit-exact)
@@ -172,6 +178,7 @@ def main():
         "hafnians_per_s": B / (ms / 1e3),
         "wall_ms_per_step": wall_ms,
         "oracle_spot_check": check,
+        "tuned_launch": tuned,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
                      "kernel": "k_level_fwd", "launches_per_step": launches,
@@ -187,6 +194,21 @@ def main():
         hip.device_synchronize()
         result["copy_calibration_bytes"] = nbytes_copy
 
+    if not args.no_grad:
+        # forward + full gradient d haf / d P for the same batch, inputs resident in HBM
+        dPd = hip.DeviceBuffer(B * 8 * (K * (K - 1) // 2))
+        h = te._device_plan(be, plan)
+
+        def grad_step():
+            _lib.check(_lib.lib().eggman_dp_forward_grad_device(
+                h, _lib.F64, Pd.addr, B, B, out.addr, dPd.addr, B, stream.handle), "grad")
+        log("batched forward + gradient")
+        gsteps = max(2, args.steps // 4)
+        msg, _ = time_device(hip, stream, grad_step, gsteps, 1)
+        result["fwd_grad"] = {"ms_per_step": msg, "hafnians_with_gradient_per_s": B / (msg / 1e3),
+                              "forward_transitions_per_s": B * nt / (msg / 1e3), "steps": gsteps}
+        del dPd
+
     if not args.no_plain:
         for name, T, gen in (("plain_f64", np.float64, lambda: synth.sym_normal(K, 3, 0, 1 / np.sqrt(K))),
                              ("plain_c64", np.complex128, lambda: synth.sym_complex_normal(K, 2, 0))):
@@ -197,12 +219,20 @@ def main():
 
             def one():
                 te._haf_dp_device(be, plan, T, P1d.addr, 1, 1, o1.addr, stream.handle)
+            choice = None
+            if not args.no_tune:
+                log(f"autotune {name}")
+                choice = te.autotune(K, T, 1, reps=5)["choice"]
             log(f"{name}")
             steps1 = max(args.steps * 5, 50)
             ms1, wall1 = time_device(hip, stream, one, steps1, args.warmup)
             result[name] = {"ms_per_hafnian": ms1, "wall_ms_per_hafnian": wall1,
-                            "hafnians_per_s": 1e3 / ms1, "transitions_per_s": nt / (ms1 / 1e3)}
+                            "hafnians_per_s": 1e3 / ms1, "transitions_per_s": nt / (ms1 / 1e3),
+                            "tuned_launch": choice}
         result["batched_vs_plain_f64_hafnians"] = result["hafnians_per_s"] / result["plain_f64"]["hafnians_per_s"]
+        if "fwd_grad" in result:
+            result["batched_grad_vs_plain_f64_hafnians"] = (
+                result["fwd_grad"]["hafnians_with_gradient_per_s"] / result["plain_f64"]["hafnians_per_s"])
 
     if not args.no_cpu:
         log("cpu baseline")

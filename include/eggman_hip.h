@@ -129,6 +129,18 @@ int eggman_dp_forward_device(uint64_t plan, int dtype, const void *P_dev, int64_
                              int64_t B, void *out_dev, void *stream);
 
 /*
+ * The boson-sampling batch shape, hafnian_repeated(A, rpts; backend) (src/hafnian.jl:1317-1359):
+ * one matrix A (N x N, column-major as Julia stores it; int64 entries for EGGMAN_I128) and B index
+ * lists idx_host[b][0..K-1] (int32, 1-based rows of A, repeats allowed, K = the plan's degree).
+ * A and the lists are copied to the device and P[q][b] = A[idx_b[i], idx_b[j]] (pair q = (i < j))
+ * is gathered there, replacing the host loop of _gather_pairs_batch (src/dp.jl:479-494) and the
+ * B x npairs host-to-device copy.  Results as eggman_dp_forward, bit-identical to it.
+ */
+int eggman_dp_forward_repeated(uint64_t plan, int dtype, const void *A_host, int64_t N,
+                               const int32_t *idx_host, int64_t B, int64_t max_batch,
+                               void *out_host);
+
+/*
  * Hafnians of instances of different degrees in one device pass.  Group g has counts[g]
  * instances sharing plan handle plans[g]; P_host is the concatenation of the groups'
  * [pair][counts[g]] blocks (group 0 first) and out_host receives the groups' results in the same
@@ -182,12 +194,13 @@ int eggman_autotune(uint64_t plan, int dtype, int64_t B, int reps, char *report,
  * Set / read a tuning entry directly (e.g. to persist tuned choices across processes).  bt:
  * lanes per state (1..64, power of two); chunk: instances per pass; lds: stage pair rows in LDS;
  * occ: workgroups per CU (0 = as LDS allows); graph: replay the level sequence from a HIP graph
- * captured on first use (reused while the same buffers are passed).
+ * captured on first use (reused while the same buffers are passed); fused: one workgroup per
+ * instance walks every level with all state values in LDS (degrees whose state array fits).
  */
 int eggman_tune_set(int K, int index_bytes, int dtype, int64_t B, int bt, int64_t chunk, int lds,
-                    int occ, int graph);
+                    int occ, int graph, int fused);
 int eggman_tune_get(int K, int index_bytes, int dtype, int64_t B, int *bt, int64_t *chunk,
-                    int *lds, int *occ, int *graph);
+                    int *lds, int *occ, int *graph, int *fused);
 
 /* ---------------------------------------------------------------- caches */
 

@@ -31,7 +31,9 @@ def test_autotune_choice_is_bit_exact(K, B):
 @pytest.mark.parametrize("setting", [dict(bt=64, chunk=64, lds=1, occ=1, graph=0),
                                      dict(bt=64, chunk=128, lds=0, occ=0, graph=0),
                                      dict(bt=32, chunk=0, lds=0, occ=0, graph=0),
-                                     dict(bt=64, chunk=0, lds=1, occ=2, graph=1)])
+                                     dict(bt=64, chunk=0, lds=1, occ=2, graph=1),
+                                     dict(bt=64, chunk=0, lds=0, occ=0, graph=0, fused=1),
+                                     dict(bt=1, chunk=0, lds=0, occ=0, graph=1, fused=1)])
 def test_every_launch_choice_gives_the_same_bits(setting):
     K, B = 18, 150
     te.tune_set(K, np.float64, B, **setting)
@@ -69,4 +71,5 @@ def test_tuning_table_round_trip(tmp_path):
     path = tmp_path / "tuning.json"
     te.save_tuning(str(path))
     assert te.load_tuning(str(path)) >= 1
-    assert te.tune_get(22, np.float64, 512) == {"bt": 64, "chunk": 256, "lds": 1, "occ": 2, "graph": 0}
+    assert te.tune_get(22, np.float64, 512) == {"bt": 64, "chunk": 256, "lds": 1, "occ": 2,
+                                                "graph": 0, "fused": 0}

@@ -79,6 +79,7 @@ struct GraphEntry {
     int64_t chunk = 0;
     bool lds = false;
     int occ = 0;
+    bool fused = false;
     uint64_t gen = 0;                // work-buffer generation the graph's addresses belong to
     hipGraphExec_t exec = nullptr;
 };
@@ -100,8 +101,11 @@ struct DevState {
     DevBuf G;                        // adjoint values (ping-pong levels)
     DevBuf part;                     // per-segment partial gradients
     DevBuf dP;                       // staged gradients (host entry points)
+    DevBuf A;                        // staged matrix (repeated-pattern batches)
+    DevBuf idx;                      // staged index lists (repeated-pattern batches)
     size_t bytes() const {
-        return H.bytes + P.bytes + out.bytes + Hg.bytes + G.bytes + part.bytes + dP.bytes;
+        return H.bytes + P.bytes + out.bytes + Hg.bytes + G.bytes + part.bytes + dP.bytes +
+               A.bytes + idx.bytes;
     }
 };
 
@@ -199,6 +203,7 @@ struct Tune {
     bool lds = true;    // stage the level's pair rows in LDS (bt == 64 only)
     int occ = 0;        // workgroups per CU for the LDS kernel (0 = as many as LDS allows, <= 4)
     bool graph = false; // replay the level sequence from a captured HIP graph
+    bool fused = false; // one workgroup per instance walks every level with H in LDS (small K)
 };
 
 static int pow2_at_least(int64_t n) {
@@ -346,6 +351,47 @@ static void launch_level_bt(int bt, const DevPlan &pl, int L, const void *P, int
     }
 }
 
+// LDS bytes of the fused kernel: every state value plus the instance's pair values.
+static size_t fused_lds_bytes(const PlanLayout &lay, int dtype) {
+    return (((size_t)lay.nstates * elt_hbytes(dtype) + 15) & ~(size_t)15) +
+           (size_t)lay.npairs * elt_pbytes(dtype);
+}
+
+// Launch the fused kernel; false (nothing launched) if the state array does not fit in LDS.
+template <int DT, typename W>
+static bool launch_fused(const DevCaps &caps, const DevPlan &pl, const void *P, int64_t ldp,
+                         int64_t B, void *out, hipStream_t st) {
+    using E = Elt<DT>;
+    const PlanLayout &lay = pl.lay;
+    const size_t lds = fused_lds_bytes(lay, DT);
+    if (lds > caps.max_lds_block || lay.nlevels - 1 > kFusedMaxLevels || lay.nstates > INT32_MAX)
+        return false;
+    auto kern = k_fused_lds<DT, W>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)caps.max_lds_block) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        attr_set = true;
+    }
+    FusedLevels lv;
+    lv.nlev = lay.K / 2 + 1;
+    for (int L = 1; L <= lv.nlev; L++) {
+        lv.first[L - 1] = (int32_t)(lay.levels[L - 1] - 1);
+        lv.nst[L - 1] = (int32_t)lay.nst[L - 1];
+        lv.tstart[L - 1] = lay.tstart[L - 1];
+        lv.k[L - 1] = std::max(0, lay.size[L - 1] - 1);
+    }
+    const int64_t occ = std::max<int64_t>(1, (int64_t)(caps.lds_per_cu / lds));
+    const int64_t blocks = std::min<int64_t>(B, (int64_t)caps.cus * occ);
+    kern<<<dim3((unsigned)blocks), dim3(1024), lds, st>>>(
+        (const W *)pl.d_trans, lv, (int)lay.nstates, (int)lay.npairs, (const typename E::P *)P, ldp,
+        (typename E::H *)out, B);
+    return true;
+}
+
 // Walk all levels for instances [0, B) of P (rows ldp apart) into out, `chunk` instances per
 // pass, keeping only two levels of state values (the level being written and the level below,
 // which holds every child it reads).
@@ -354,6 +400,10 @@ static int forward_levels(DevState &ds, const DevPlan &pl, const void *P, int64_
                           void *out, hipStream_t st, const Tune &tune) {
     using E = Elt<DT>;
     const PlanLayout &lay = pl.lay;
+    if (tune.fused && launch_fused<DT, W>(ds.caps, pl, P, ldp, B, out, st)) {
+        EGG_HIP(hipGetLastError());
+        return EGGMAN_OK;
+    }
     const int64_t chunk = tune.chunk > 0 ? std::min(tune.chunk, B) : B;
     const size_t hb = sizeof(typename E::H), pb = sizeof(typename E::P);
     const int64_t rows = std::max<int64_t>(lay.max_level, 1);
@@ -408,7 +458,7 @@ static int forward_graph(int dtype, DevState &ds, const DevPlan &pl, const void 
         if (it->gen != g_buf_gen) continue;
         if (it->plan == pl.id && it->dtype == dtype && it->P == P && it->ldp == ldp && it->B == B &&
             it->out == out && it->bt == tune.bt && it->chunk == tune.chunk && it->lds == tune.lds &&
-            it->occ == tune.occ) {
+            it->occ == tune.occ && it->fused == tune.fused) {
             EGG_HIP(hipGraphLaunch(it->exec, st));
             return EGGMAN_OK;
         }
@@ -436,6 +486,7 @@ static int forward_graph(int dtype, DevState &ds, const DevPlan &pl, const void 
     ge.chunk = tune.chunk;
     ge.lds = tune.lds;
     ge.occ = tune.occ;
+    ge.fused = tune.fused;
     ge.gen = g_buf_gen;
     const hipError_t ei = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
@@ -800,6 +851,28 @@ static int alloc_plan(DevPlan &p) {
     return EGGMAN_OK;
 }
 
+// ------------------------------------------------------------------------------ device gather
+
+template <typename PT>
+static void launch_gather(const void *A, int64_t N, const int32_t *idx, int K, int64_t B, void *P,
+                          int64_t ldp, hipStream_t st) {
+    const int64_t total = (int64_t)K * (K - 1) / 2 * B;
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (total + 255) / 256));
+    k_gather_pairs<PT><<<dim3((unsigned)blocks), dim3(256), 0, st>>>((const PT *)A, N, idx, K, B,
+                                                                      (PT *)P, ldp);
+}
+
+static void gather_any(int dtype, const void *A, int64_t N, const int32_t *idx, int K, int64_t B,
+                       void *P, int64_t ldp, hipStream_t st) {
+    switch (dtype) {
+    case EGGMAN_F32: launch_gather<float>(A, N, idx, K, B, P, ldp, st); break;
+    case EGGMAN_F64: launch_gather<double>(A, N, idx, K, B, P, ldp, st); break;
+    case EGGMAN_C32: launch_gather<float2>(A, N, idx, K, B, P, ldp, st); break;
+    case EGGMAN_C64: launch_gather<double2>(A, N, idx, K, B, P, ldp, st); break;
+    case EGGMAN_I128: launch_gather<long long>(A, N, idx, K, B, P, ldp, st); break;
+    }
+}
+
 }  // namespace eggman
 
 using namespace eggman;
@@ -1038,6 +1111,12 @@ int eggman_autotune(uint64_t plan, int dtype, int64_t B, int reps, char *report,
         if (bt != 64) cands.push_back(Tune{64, B, false, 0});
         cands.push_back(Tune{64, B, true, 0});
     }
+    if (fused_lds_bytes(pl->lay, dtype) <= ds->caps.max_lds_block) {
+        Tune f;
+        f.fused = true;
+        f.chunk = B;
+        cands.push_back(f);
+    }
     DevBuf Pd, Od;
     EGG_TRY(ensure(Pd, (size_t)B * pl->lay.npairs * pb));
     EGG_TRY(ensure(Od, (size_t)B * hb));
@@ -1068,8 +1147,8 @@ int eggman_autotune(uint64_t plan, int dtype, int64_t B, int reps, char *report,
         }
         ms /= reps;
         char line[160];
-        snprintf(line, sizeof line, "bt=%d chunk=%lld lds=%d occ=%d graph=%d ms=%.4f\n", t.bt,
-                 (long long)t.chunk, (int)t.lds, t.occ, (int)t.graph, ms);
+        snprintf(line, sizeof line, "bt=%d chunk=%lld lds=%d occ=%d graph=%d fused=%d ms=%.4f\n",
+                 t.bt, (long long)t.chunk, (int)t.lds, t.occ, (int)t.graph, (int)t.fused, ms);
         rep += line;
         if (ms < best) {
             best = ms;
@@ -1090,8 +1169,9 @@ int eggman_autotune(uint64_t plan, int dtype, int64_t B, int reps, char *report,
     if (rc != EGGMAN_OK) return rc;
     g_tune[TuneKey(pl->device, pl->lay.K, pl->lay.index_bytes, dtype, batch_bucket(B))] = best_t;
     char line[160];
-    snprintf(line, sizeof line, "best bt=%d chunk=%lld lds=%d occ=%d graph=%d ms=%.4f\n", best_t.bt,
-             (long long)best_t.chunk, (int)best_t.lds, best_t.occ, (int)best_t.graph, best);
+    snprintf(line, sizeof line, "best bt=%d chunk=%lld lds=%d occ=%d graph=%d fused=%d ms=%.4f\n",
+             best_t.bt, (long long)best_t.chunk, (int)best_t.lds, best_t.occ, (int)best_t.graph,
+             (int)best_t.fused, best);
     rep += line;
     if (report && report_len > 0) {
         const size_t n = std::min<size_t>(rep.size(), (size_t)report_len - 1);
@@ -1102,19 +1182,19 @@ int eggman_autotune(uint64_t plan, int dtype, int64_t B, int reps, char *report,
 }
 
 int eggman_tune_set(int K, int index_bytes, int dtype, int64_t B, int bt, int64_t chunk, int lds,
-                    int occ, int graph) {
+                    int occ, int graph, int fused) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     if (!elt_hbytes(dtype) || B < 1 || (bt & (bt - 1)) || bt < 1 || bt > 64 || occ < 0)
         return fail(EGGMAN_EINVAL, "bad tuning entry");
     int dev;
     EGG_TRY(current_device(&dev));
     g_tune[TuneKey(dev, K, index_bytes, dtype, batch_bucket(B))] =
-        Tune{bt, chunk, lds != 0, occ, graph != 0};
+        Tune{bt, chunk, lds != 0, occ, graph != 0, fused != 0};
     return EGGMAN_OK;
 }
 
 int eggman_tune_get(int K, int index_bytes, int dtype, int64_t B, int *bt, int64_t *chunk,
-                    int *lds, int *occ, int *graph) {
+                    int *lds, int *occ, int *graph, int *fused) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     int dev;
     EGG_TRY(current_device(&dev));
@@ -1125,6 +1205,49 @@ int eggman_tune_get(int K, int index_bytes, int dtype, int64_t B, int *bt, int64
     if (lds) *lds = it->second.lds;
     if (occ) *occ = it->second.occ;
     if (graph) *graph = it->second.graph;
+    if (fused) *fused = it->second.fused;
+    return EGGMAN_OK;
+}
+
+int eggman_dp_forward_repeated(uint64_t plan, int dtype, const void *A_host, int64_t N,
+                               const int32_t *idx_host, int64_t B, int64_t max_batch,
+                               void *out_host) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    DevPlan *pl;
+    EGG_TRY(find_plan(plan, &pl));
+    const int hb = elt_hbytes(dtype), pb = elt_pbytes(dtype);
+    if (!hb) return fail(EGGMAN_EINVAL, "unknown dtype %d", dtype);
+    if (B < 0 || N < 1) return fail(EGGMAN_EINVAL, "need N >= 1 and B >= 0");
+    if (B == 0) return EGGMAN_OK;
+    if (!A_host || !idx_host || !out_host) return fail(EGGMAN_EINVAL, "null A, idx or out pointer");
+    const int K = pl->lay.K;
+    for (int64_t t = 0; t < B * K; t++)
+        if (idx_host[t] < 1 || idx_host[t] > N)
+            return fail(EGGMAN_EINVAL, "index list %lld holds %d, outside 1..%lld",
+                        (long long)(t / K + 1), idx_host[t], (long long)N);
+    DeviceGuard g(pl->device);
+    EGG_TRY(g.rc);
+    DevState *ds;
+    EGG_TRY(dev_state(pl->device, &ds));
+    EGG_TRY(chain_stream(*ds, ds->stream));
+    const int64_t chunk = max_batch > 0 ? std::min(max_batch, B) : B;
+    const int64_t npairs = pl->lay.npairs;
+    EGG_TRY(ensure(ds->P, (size_t)chunk * npairs * pb));
+    EGG_TRY(ensure(ds->out, (size_t)chunk * hb));
+    EGG_TRY(ensure(ds->A, (size_t)N * N * pb));
+    EGG_TRY(ensure(ds->idx, (size_t)chunk * K * sizeof(int32_t)));
+    EGG_HIP(hipMemcpyAsync(ds->A.p, A_host, (size_t)N * N * pb, hipMemcpyHostToDevice, ds->stream));
+    const Tune tune = pick_tune(*pl, dtype, chunk);
+    for (int64_t b0 = 0; b0 < B; b0 += chunk) {
+        const int64_t nb = std::min(chunk, B - b0);
+        EGG_HIP(hipMemcpyAsync(ds->idx.p, idx_host + b0 * K, (size_t)nb * K * sizeof(int32_t),
+                               hipMemcpyHostToDevice, ds->stream));
+        gather_any(dtype, ds->A.p, N, (const int32_t *)ds->idx.p, K, nb, ds->P.p, nb, ds->stream);
+        EGG_TRY(forward_any(dtype, *ds, *pl, ds->P.p, nb, nb, ds->out.p, ds->stream, tune));
+        EGG_HIP(hipMemcpyAsync((char *)out_host + (size_t)b0 * hb, ds->out.p, (size_t)nb * hb,
+                               hipMemcpyDeviceToHost, ds->stream));
+        EGG_HIP(hipStreamSynchronize(ds->stream));
+    }
     return EGGMAN_OK;
 }
 
@@ -1278,7 +1401,8 @@ int eggman_cache_clear(void) {
         for (GraphEntry &ge : kv.second.graphs) (void)hipGraphExecDestroy(ge.exec);
         kv.second.graphs.clear();
         for (DevBuf *b : {&kv.second.H, &kv.second.P, &kv.second.out, &kv.second.Hg,
-                          &kv.second.G, &kv.second.part, &kv.second.dP}) {
+                          &kv.second.G, &kv.second.part, &kv.second.dP, &kv.second.A,
+                          &kv.second.idx}) {
             if (b->p) (void)hipFree(b->p);
             b->p = nullptr;
             b->bytes = 0;

@@ -197,6 +197,88 @@ __global__ __launch_bounds__(WPB * 64) void k_level_fwd_lds_multi(MultiLevelArgs
                                reinterpret_cast<typename Elt<DT>::P *>(lds_raw));
 }
 
+// ---------------------------------------------------------------------------------------
+// Whole walk of one instance in one workgroup, every state value held in LDS.
+//
+// For small degrees the whole state array fits in LDS (10946 states, 86 KB of FP64 at K = 20),
+// so one workgroup can run every level of an instance with a workgroup barrier between levels
+// instead of a kernel launch: the plain path then costs a single launch.  Instances are spread
+// over workgroups (grid-stride); each state is still summed by one lane in plan order.
+constexpr int kFusedMaxLevels = 33;
+
+struct FusedLevels {
+    int nlev;                           // levels 2 .. nlev (1-based), level 1 is the empty set
+    int32_t first[kFusedMaxLevels + 1]; // 0-based first state of level L at [L-1]
+    int32_t nst[kFusedMaxLevels + 1];
+    int64_t tstart[kFusedMaxLevels + 1];
+    int32_t k[kFusedMaxLevels + 1];
+};
+
+template <int DT, typename W>
+__global__ __launch_bounds__(1024) void k_fused_lds(
+    const W *__restrict__ trans, FusedLevels lv, int nstates, int npairs,
+    const typename Elt<DT>::P *__restrict__ P, int64_t ldp,
+    typename Elt<DT>::H *__restrict__ out, int64_t B) {
+    using E = Elt<DT>;
+    using HT = typename E::H;
+    using PT = typename E::P;
+    constexpr int bits = Packing<W>::bits;
+    constexpr W mask = (W)((W(1) << bits) - 1);
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    HT *Hs = reinterpret_cast<HT *>(lds_raw);
+    PT *Ps = reinterpret_cast<PT *>(lds_raw + (((size_t)nstates * sizeof(HT) + 15) & ~(size_t)15));
+    for (int64_t b = blockIdx.x; b < B; b += gridDim.x) {
+        for (int q = threadIdx.x; q < npairs; q += blockDim.x) Ps[q] = P[(int64_t)q * ldp + b];
+        if (threadIdx.x == 0) Hs[0] = E::one();
+        __syncthreads();
+        for (int L = 2; L <= lv.nlev; L++) {
+            const int first = lv.first[L - 1], n = lv.nst[L - 1], k = lv.k[L - 1];
+            const W *tl = trans + lv.tstart[L - 1];
+            for (int r = threadIdx.x; r < n; r += blockDim.x) {
+                const W *tw = tl + (int64_t)r * k;
+                HT acc = E::zero();
+#pragma unroll 4
+                for (int i = 0; i < k; i++) {
+                    const W t = tw[i];
+                    acc = E::muladd(acc, Ps[(int)(t & mask) - 1], Hs[(int64_t)(t >> bits) - 1]);
+                }
+                Hs[first + r] = acc;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[b] = Hs[nstates - 1];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Pair values gathered on the device for the boson-sampling batch shape: one matrix A, many
+// index lists (src/dp.jl:479-494).  P[q][b] = A[idx_b[i], idx_b[j]] for the q-th pair i < j in
+// the row-major order of _pair_index (src/unrolled.jl:60).  A is column-major (Julia's layout):
+// element (r, c) at c*N + r, 0-based.  One thread per (pair, instance), instance fastest, so the
+// writes are coalesced.
+template <typename PT>
+__global__ __launch_bounds__(256) void k_gather_pairs(const PT *__restrict__ A, int64_t N,
+                                                      const int32_t *__restrict__ idx, int K,
+                                                      int64_t B, PT *__restrict__ P, int64_t ldp) {
+    const int64_t npairs = (int64_t)K * (K - 1) / 2;
+    const int64_t total = npairs * B;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride) {
+        const int64_t q = t / B, b = t - q * B;
+        // row i (0-based) of the upper triangle holds pairs [i*K - i(i+1)/2, ...) of length K-1-i
+        int i = 0;
+        int64_t base = 0;
+        while (base + (K - 1 - i) <= q) {
+            base += K - 1 - i;
+            i++;
+        }
+        const int j = i + 1 + (int)(q - base);
+        const int64_t r = idx[b * K + i] - 1, c = idx[b * K + j] - 1;
+        P[q * ldp + b] = A[c * N + r];
+    }
+}
+
 // Row of ones for the empty set (H[1] = one, src/dp.jl:421): a plain store kernel, so the
 // sequence stays valid inside a captured graph.
 template <int DT>

@@ -152,6 +152,31 @@ def _haf_dp_backend_grad(backend: HIPBackend, P: np.ndarray, plan: HafnianDPPlan
     return out, dP.T
 
 
+def _haf_dp_backend_repeated(backend: HIPBackend, A: np.ndarray, idxs, plan, *,
+                             max_batch: int | None = None):
+    """The boson-sampling batch: hafnians of A[idx_b, idx_b] for every 1-based index list idx_b
+    (all of length plan.K), with the pair values gathered on the device from one copy of A.
+    Returns a length-B array of A's element type (lists of ints for int64 A).  Bit-identical to
+    gathering on the host and calling _haf_dp_backend."""
+    if not isinstance(backend, HIPBackend):
+        raise ArgumentError(f"no GPU backend method for {type(backend).__name__}")
+    A = np.asfortranarray(A)                      # column-major, as Julia stores it
+    N = A.shape[0]
+    code = dtype_code(A.dtype)
+    idx = np.ascontiguousarray(np.asarray(idxs, dtype=np.int32).reshape(-1, plan.K))
+    B = idx.shape[0]
+    if B == 0:
+        return [] if code == I128 else np.empty(0, A.dtype)
+    h = _device_plan(backend, plan)
+    out = np.empty((B, 2), np.uint64) if code == I128 else np.empty(B, A.dtype)
+    mb = 0 if max_batch is None else max(1, min(B, int(max_batch)))
+    check(lib().eggman_dp_forward_repeated(h, code, A.ctypes.data, N, idx.ctypes.data, B, mb,
+                                           out.ctypes.data), "dp_forward_repeated")
+    if code == I128:
+        return [_limbs_to_int(int(lo), int(hi)) for lo, hi in out]
+    return out
+
+
 def _haf_dp_backend_mixed(backend: HIPBackend, Ps, plans):
     """Hafnians of several groups of instances, group g of degree plans[g].K with pair values
     Ps[g] (shape (B_g, npairs_g)), in one device pass that advances every group by one level per

@@ -12,7 +12,7 @@ import numpy as np
 
 from ._lib import ArgumentError, DimensionMismatch, I128
 from .backend import (HIPBackend, _haf_dp_backend, _haf_dp_backend_grad,
-                      _haf_dp_backend_mixed, dtype_code)
+                      _haf_dp_backend_mixed, _haf_dp_backend_repeated, dtype_code)
 from .plan import DP_HARD_MAX, DP_MAX, UNROLL_MAX, PlanRef, _dp_plan
 
 _DEFAULT_BACKEND = None
@@ -275,11 +275,10 @@ def _hafnian_repeated_batch(A, rpts, *, method, check_symmetric, backend, max_ba
     if method in ("unrolled", "sieve"):
         _route(method, total)
     # the batched repeated form sends every total up to DP_HARD_MAX to the device under :auto
-    # (src/hafnian.jl:1348)
-    idxs = [_expanded_indices(np.asarray(r, dtype=np.int64), total) for r in rpts]
-    P = _gather_pairs_batch(T, A, idxs, total)
-    return _haf_dp_backend(_backend_or_default(backend), P, _plan_for(total, backend),
-                           max_batch=max_batch)
+    # (src/hafnian.jl:1348); A and the index lists go to the device, which gathers P itself
+    idxs = np.stack([_expanded_indices(np.asarray(r, dtype=np.int64), total) for r in rpts])
+    return _haf_dp_backend_repeated(_backend_or_default(backend), A.astype(T, copy=False), idxs,
+                                    _plan_for(total, backend), max_batch=max_batch)
 
 
 def hafnian_exact(A, *, check_symmetric: bool = True, backend=None,

@@ -17,7 +17,7 @@ from ._lib import check, lib
 from .backend import HIPBackend, _device_plan, dtype_code
 from .plan import DP_MAX, PlanRef, _dp_plan
 
-_FIELDS = ("bt", "chunk", "lds", "occ", "graph")
+_FIELDS = ("bt", "chunk", "lds", "occ", "graph", "fused")
 _TUNED: dict = {}   # (K, index_bytes, dtype code, B) -> choice, for save_tuning
 
 
@@ -47,22 +47,24 @@ def autotune(K: int, dtype=np.float64, B: int = 1024, reps: int = 3,
 def tune_get(K: int, dtype=np.float64, B: int = 1024):
     """The tuning entry the library uses for (K, dtype, B), or None."""
     ib = 4 if K <= DP_MAX else 8
-    vals = [ctypes.c_int() for _ in range(4)]
+    vals = [ctypes.c_int() for _ in range(5)]
     chunk = ctypes.c_int64()
     rc = lib().eggman_tune_get(K, ib, dtype_code(dtype), B, ctypes.byref(vals[0]),
                                ctypes.byref(chunk), ctypes.byref(vals[1]), ctypes.byref(vals[2]),
-                               ctypes.byref(vals[3]))
+                               ctypes.byref(vals[3]), ctypes.byref(vals[4]))
     if rc != 0:
         return None
     return {"bt": vals[0].value, "chunk": chunk.value, "lds": vals[1].value,
-            "occ": vals[2].value, "graph": vals[3].value}
+            "occ": vals[2].value, "graph": vals[3].value, "fused": vals[4].value}
 
 
-def tune_set(K: int, dtype, B: int, bt: int, chunk: int, lds: int, occ: int, graph: int = 0) -> None:
+def tune_set(K: int, dtype, B: int, bt: int, chunk: int, lds: int, occ: int, graph: int = 0,
+             fused: int = 0) -> None:
     ib = 4 if K <= DP_MAX else 8
     code = dtype_code(dtype)
-    check(lib().eggman_tune_set(K, ib, code, B, bt, chunk, lds, occ, graph), "tune_set")
-    _TUNED[(K, ib, code, B)] = {"bt": bt, "chunk": chunk, "lds": lds, "occ": occ, "graph": graph}
+    check(lib().eggman_tune_set(K, ib, code, B, bt, chunk, lds, occ, graph, fused), "tune_set")
+    _TUNED[(K, ib, code, B)] = {"bt": bt, "chunk": chunk, "lds": lds, "occ": occ, "graph": graph,
+                                "fused": fused}
 
 
 def save_tuning(path: str) -> None:
@@ -77,6 +79,7 @@ def load_tuning(path: str) -> int:
         rows = json.load(f)
     for r in rows:
         check(lib().eggman_tune_set(r["K"], r["index_bytes"], r["dtype"], r["B"], r["bt"],
-                                    r["chunk"], r["lds"], r["occ"], r.get("graph", 0)), "tune_set")
+                                    r["chunk"], r["lds"], r["occ"], r.get("graph", 0),
+                                    r.get("fused", 0)), "tune_set")
         _TUNED[(r["K"], r["index_bytes"], r["dtype"], r["B"])] = {k: r.get(k, 0) for k in _FIELDS}
     return len(rows)
