@@ -104,7 +104,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-path lines")
     ap.add_argument("--no-grad", action="store_true", help="skip the forward+gradient line")
-    ap.add_argument("--no-tune", action="store_true", help="use the default launch choices")
+    ap.add_argument("--autotune", action="store_true",
+                    help="re-run the autotuner before timing instead of using the shipped table")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="ignore every tuning table: built-in default launch choices")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="after the timed steps, run one device-to-device copy of this many MiB "
                          "(a kernel of known traffic, to scale the HBM counters of a profile)")
@@ -135,10 +138,13 @@ def main():
     def step():
         te._haf_dp_device(be, plan, np.float64, Pd.addr, B, B, out.addr, stream.handle)
 
-    tuned = None
-    if not args.no_tune:
+    if args.no_tune:
+        te.tune_set(K, np.float64, B, bt=64 if B >= 64 else 1 << max(0, (B - 1).bit_length()),
+                    chunk=B, occ=0, lds=True)
+    elif args.autotune:
         log("autotune batched")
-        tuned = te.autotune(K, np.float64, B, reps=2)["choice"]
+        te.autotune(K, np.float64, B, reps=2)
+    tuned = te.tune_get(K, np.float64, B)
     log("batched forward")
     ms, wall_ms = time_device(hip, stream, step, args.steps, args.warmup)
     # spot-check the timed result against the oracle (bit-exact)
@@ -157,7 +163,10 @@ def main():
 
     value = B * nt / (ms / 1e3)
     nbytes = alg_bytes_fwd(K, B, 8, ib, ns, nt)
-    launches = K // 2
+    # level launches per step: K/2 levels for every pass of `chunk` instances
+    chunk = (tuned or {}).get("chunk") or B
+    launches = (K // 2) * (-(-B // min(chunk, B)))
+    head = "k_level_fwd_lds" if (tuned or {}).get("lds", 1) and not (tuned or {}).get("fused") else "k_level_fwd"
     achieved = nbytes / (ms / 1e3) / 1e9
     result = {
         "metric": "DP transitions/s (batched N=28 FP64 forward)",
@@ -181,7 +190,7 @@ def main():
         "tuned_launch": tuned,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
-                     "kernel": "k_level_fwd", "launches_per_step": launches,
+                     "kernel": head, "launches_per_step": launches,
                      "alg_bytes_per_launch": nbytes / launches,
                      "avg_launch_ms": ms / launches},
     }
@@ -219,10 +228,12 @@ def main():
 
             def one():
                 te._haf_dp_device(be, plan, T, P1d.addr, 1, 1, o1.addr, stream.handle)
-            choice = None
-            if not args.no_tune:
+            if args.no_tune:
+                te.tune_set(K, T, 1, bt=1, chunk=1, occ=0, lds=False)
+            elif args.autotune:
                 log(f"autotune {name}")
-                choice = te.autotune(K, T, 1, reps=5)["choice"]
+                te.autotune(K, T, 1, reps=5)
+            choice = te.tune_get(K, T, 1)
             log(f"{name}")
             steps1 = max(args.steps * 5, 50)
             ms1, wall1 = time_device(hip, stream, one, steps1, args.warmup)

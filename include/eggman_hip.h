@@ -190,17 +190,23 @@ int eggman_dp_forward_grad_device(uint64_t plan, int dtype, const void *P_dev, i
 int eggman_autotune(uint64_t plan, int dtype, int64_t B, int reps, char *report,
                     int64_t report_len);
 
+/* Tuning flags */
+#define EGGMAN_TUNE_LDS 1   /* stage the level's pair rows in LDS (64 lanes per state) */
+#define EGGMAN_TUNE_GRAPH 2 /* replay the level sequence from a HIP graph captured on first use,
+                               reused while the same buffers are passed */
+#define EGGMAN_TUNE_FUSED 4 /* one workgroup per instance walks every level with all state
+                               values in LDS (degrees whose state array fits) */
+#define EGGMAN_TUNE_WIDE 8  /* 16 waves per workgroup in the LDS kernel instead of 8 */
+
 /*
  * Set / read a tuning entry directly (e.g. to persist tuned choices across processes).  bt:
- * lanes per state (1..64, power of two); chunk: instances per pass; lds: stage pair rows in LDS;
- * occ: workgroups per CU (0 = as LDS allows); graph: replay the level sequence from a HIP graph
- * captured on first use (reused while the same buffers are passed); fused: one workgroup per
- * instance walks every level with all state values in LDS (degrees whose state array fits).
+ * lanes per state (1..64, power of two); chunk: instances per pass over the levels; flags:
+ * EGGMAN_TUNE_*; occ: workgroups per CU for the LDS kernel (0 = as many as LDS allows).
  */
-int eggman_tune_set(int K, int index_bytes, int dtype, int64_t B, int bt, int64_t chunk, int lds,
-                    int occ, int graph, int fused);
+int eggman_tune_set(int K, int index_bytes, int dtype, int64_t B, int bt, int64_t chunk,
+                    int flags, int occ);
 int eggman_tune_get(int K, int index_bytes, int dtype, int64_t B, int *bt, int64_t *chunk,
-                    int *lds, int *occ, int *graph, int *fused);
+                    int *flags, int *occ);
 
 /* ---------------------------------------------------------------- caches */
 

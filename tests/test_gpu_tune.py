@@ -28,11 +28,12 @@ def test_autotune_choice_is_bit_exact(K, B):
     assert got.tobytes() == want.tobytes()
 
 
-@pytest.mark.parametrize("setting", [dict(bt=64, chunk=64, lds=1, occ=1, graph=0),
-                                     dict(bt=64, chunk=128, lds=0, occ=0, graph=0),
-                                     dict(bt=32, chunk=0, lds=0, occ=0, graph=0),
+@pytest.mark.parametrize("setting", [dict(bt=64, chunk=64, lds=1, occ=1),
+                                     dict(bt=64, chunk=64, lds=1, occ=2, wide=1),
+                                     dict(bt=64, chunk=128, lds=0, occ=0),
+                                     dict(bt=32, chunk=0, lds=0, occ=0),
                                      dict(bt=64, chunk=0, lds=1, occ=2, graph=1),
-                                     dict(bt=64, chunk=0, lds=0, occ=0, graph=0, fused=1),
+                                     dict(bt=64, chunk=0, lds=0, occ=0, fused=1),
                                      dict(bt=1, chunk=0, lds=0, occ=0, graph=1, fused=1)])
 def test_every_launch_choice_gives_the_same_bits(setting):
     K, B = 18, 150
@@ -45,7 +46,7 @@ def test_every_launch_choice_gives_the_same_bits(setting):
 
 def test_graph_replay_on_device_buffers():
     K, B = 20, 1
-    te.tune_set(K, np.complex128, B, bt=1, chunk=1, lds=0, occ=0, graph=1)
+    te.tune_set(K, np.complex128, B, bt=1, chunk=1, occ=0, lds=False, graph=True)
     A = synth.sym_complex_normal(K, 62, 0)
     P = synth.pair_batch(A[None])
     Pd = hip.DeviceBuffer.from_array(P)
@@ -67,9 +68,9 @@ def test_graph_replay_on_device_buffers():
 
 
 def test_tuning_table_round_trip(tmp_path):
-    te.tune_set(22, np.float64, 512, bt=64, chunk=256, lds=1, occ=2, graph=0)
+    te.tune_set(22, np.float64, 512, bt=64, chunk=256, occ=2, lds=True, wide=True)
     path = tmp_path / "tuning.json"
     te.save_tuning(str(path))
     assert te.load_tuning(str(path)) >= 1
-    assert te.tune_get(22, np.float64, 512) == {"bt": 64, "chunk": 256, "lds": 1, "occ": 2,
-                                                "graph": 0, "fused": 0}
+    assert te.tune_get(22, np.float64, 512) == {"bt": 64, "chunk": 256, "flags": 1 | 8, "occ": 2,
+                                                "lds": 1, "graph": 0, "fused": 0, "wide": 1}

@@ -79,7 +79,7 @@ struct GraphEntry {
     int64_t chunk = 0;
     bool lds = false;
     int occ = 0;
-    bool fused = false;
+    bool fused = false, wide = false;
     uint64_t gen = 0;                // work-buffer generation the graph's addresses belong to
     hipGraphExec_t exec = nullptr;
 };
@@ -204,7 +204,15 @@ struct Tune {
     int occ = 0;        // workgroups per CU for the LDS kernel (0 = as many as LDS allows, <= 4)
     bool graph = false; // replay the level sequence from a captured HIP graph
     bool fused = false; // one workgroup per instance walks every level with H in LDS (small K)
+    bool wide = false;  // 16 waves per workgroup in the LDS kernel instead of 8
 };
+
+// Tuning flags as they cross the C ABI (eggman_tune_set / eggman_tune_get).
+enum { TUNE_LDS = 1, TUNE_GRAPH = 2, TUNE_FUSED = 4, TUNE_WIDE = 8 };
+static int tune_flags(const Tune &t) {
+    return (t.lds ? TUNE_LDS : 0) | (t.graph ? TUNE_GRAPH : 0) | (t.fused ? TUNE_FUSED : 0) |
+           (t.wide ? TUNE_WIDE : 0);
+}
 
 static int pow2_at_least(int64_t n) {
     int b = 1;
@@ -257,13 +265,14 @@ static void launch_level(const DevPlan &pl, int L, const void *P, int64_t ldp, c
         ldh, (typename E::H *)Ho, ldo, nb, ntiles);
 }
 
-constexpr int kLdsWaves = 8;   // waves per workgroup of the LDS kernel
+constexpr int kLdsWaves = 8;   // waves per workgroup of the LDS kernels (16 with Tune::wide)
 
 // Workgroup-to-(tile, states) mapping of the LDS kernels for `nst` states of `nb` instances:
-// enough workgroups per XCD to fill its CUs at the occupancy the LDS use allows, and no more
-// than there are states.
-static LdsMap make_lds_map(const DevCaps &caps, int occ_cap, int64_t nst, int64_t nb, size_t lds) {
-    int occ = (int)std::max<size_t>(1, std::min<size_t>(4, caps.lds_per_cu / std::max<size_t>(lds, 1)));
+// enough workgroups per XCD to fill its CUs at the occupancy the LDS use allows (and at most 32
+// waves per CU), and no more than there are states.
+static LdsMap make_lds_map(const DevCaps &caps, int occ_cap, int64_t nst, int64_t nb, size_t lds,
+                           int wpb = kLdsWaves) {
+    int occ = (int)std::max<size_t>(1, std::min<size_t>(32 / wpb, caps.lds_per_cu / std::max<size_t>(lds, 1)));
     if (occ_cap > 0) occ = std::min(occ, occ_cap);
     const int cus_xcd = std::max(1, caps.cus / caps.xcds);
     const int64_t target = (int64_t)cus_xcd * occ;
@@ -272,13 +281,13 @@ static LdsMap make_lds_map(const DevCaps &caps, int occ_cap, int64_t nst, int64_
     if (map.ntiles >= 8) {
         map.tiles_xcd = (map.ntiles + 7) / 8;
         int64_t bpt = std::max<int64_t>(1, target / map.tiles_xcd);
-        bpt = std::min<int64_t>(bpt, std::max<int64_t>(1, (nst + kLdsWaves - 1) / kLdsWaves));
+        bpt = std::min<int64_t>(bpt, std::max<int64_t>(1, (nst + wpb - 1) / wpb));
         map.per_xcd = map.tiles_xcd * bpt;
     } else {
         map.tiles_xcd = 0;
         const int64_t groups = (8 + map.ntiles - 1) / map.ntiles;
         const int64_t per_group = (nst + groups - 1) / groups;
-        map.per_xcd = std::max<int64_t>(1, std::min<int64_t>(target, (per_group + kLdsWaves - 1) / kLdsWaves));
+        map.per_xcd = std::max<int64_t>(1, std::min<int64_t>(target, (per_group + wpb - 1) / wpb));
     }
     return map;
 }
@@ -294,18 +303,19 @@ static void level_pairs(const PlanLayout &lay, int L, int *qlo, int *nq) {
 }
 
 // Launch one level with the LDS kernel; returns false (launching nothing) when the level's pair
-// rows do not fit in one workgroup's LDS.
-template <int DT, typename W>
-static bool launch_level_lds(const DevCaps &caps, const Tune &tune, const DevPlan &pl, int L,
-                             const void *P, int64_t ldp, const void *Hc, int64_t ldh, void *Ho,
-                             int64_t ldo, int64_t nb, hipStream_t st) {
+// rows do not fit in one workgroup's LDS.  TAG only separates the instantiation used by the
+// gradient's forward pass from the plain forward in profiles; the code is the same.
+template <int DT, typename W, int WPB, int TAG>
+static bool launch_level_lds_w(const DevCaps &caps, const Tune &tune, const DevPlan &pl, int L,
+                               const void *P, int64_t ldp, const void *Hc, int64_t ldh, void *Ho,
+                               int64_t ldo, int64_t nb, hipStream_t st) {
     using E = Elt<DT>;
     const PlanLayout &lay = pl.lay;
     int qlo, nq;
     level_pairs(lay, L, &qlo, &nq);
     const size_t lds = (size_t)nq * 64 * sizeof(typename E::P);
     if (lds > caps.max_lds_block) return false;
-    auto kern = k_level_fwd_lds<DT, W, kLdsWaves>;
+    auto kern = k_level_fwd_lds<DT, W, WPB, TAG>;
     static bool attr_set = false;
     if (!attr_set) {
         if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -316,7 +326,7 @@ static bool launch_level_lds(const DevCaps &caps, const Tune &tune, const DevPla
         attr_set = true;
     }
     const int64_t nst = lay.nst[L - 1];
-    const LdsMap map = make_lds_map(caps, tune.occ, nst, nb, lds);
+    const LdsMap map = make_lds_map(caps, tune.occ, nst, nb, lds, WPB);
     LevelArgs<DT, W> a;
     a.trans_lvl = (const W *)pl.d_trans + lay.tstart[L - 1];
     a.nst = nst;
@@ -332,8 +342,17 @@ static bool launch_level_lds(const DevCaps &caps, const Tune &tune, const DevPla
     a.ldo = ldo;
     a.nb = nb;
     a.map = map;
-    kern<<<dim3((unsigned)(8 * map.per_xcd)), dim3(kLdsWaves * 64), lds, st>>>(a);
+    kern<<<dim3((unsigned)(8 * map.per_xcd)), dim3(WPB * 64), lds, st>>>(a);
     return true;
+}
+
+template <int DT, typename W, int TAG = 0>
+static bool launch_level_lds(const DevCaps &caps, const Tune &tune, const DevPlan &pl, int L,
+                             const void *P, int64_t ldp, const void *Hc, int64_t ldh, void *Ho,
+                             int64_t ldo, int64_t nb, hipStream_t st) {
+    if (tune.wide)
+        return launch_level_lds_w<DT, W, 16, TAG>(caps, tune, pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st);
+    return launch_level_lds_w<DT, W, kLdsWaves, TAG>(caps, tune, pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st);
 }
 
 template <int DT, typename W>
@@ -458,7 +477,7 @@ static int forward_graph(int dtype, DevState &ds, const DevPlan &pl, const void 
         if (it->gen != g_buf_gen) continue;
         if (it->plan == pl.id && it->dtype == dtype && it->P == P && it->ldp == ldp && it->B == B &&
             it->out == out && it->bt == tune.bt && it->chunk == tune.chunk && it->lds == tune.lds &&
-            it->occ == tune.occ && it->fused == tune.fused) {
+            it->occ == tune.occ && it->fused == tune.fused && it->wide == tune.wide) {
             EGG_HIP(hipGraphLaunch(it->exec, st));
             return EGGMAN_OK;
         }
@@ -487,6 +506,7 @@ static int forward_graph(int dtype, DevState &ds, const DevPlan &pl, const void 
     ge.lds = tune.lds;
     ge.occ = tune.occ;
     ge.fused = tune.fused;
+    ge.wide = tune.wide;
     ge.gen = g_buf_gen;
     const hipError_t ei = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
@@ -759,7 +779,7 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
         for (int L = 2; L <= top; L++) {
             const void *Hc = level_rows(L - 1);
             void *Ho = L == top ? (void *)outc : (void *)level_rows(L);
-            if (!(tune.lds && launch_level_lds<DT, W>(ds.caps, tune, pl, L, Pc, ldp, Hc, nb, Ho, nb, nb, st)))
+            if (!(tune.lds && launch_level_lds<DT, W, 1>(ds.caps, tune, pl, L, Pc, ldp, Hc, nb, Ho, nb, nb, st)))
                 launch_level_bt<DT, W>(64, pl, L, Pc, ldp, Hc, nb, Ho, nb, nb, st);
         }
         // reverse: G of the full set is one
@@ -1102,14 +1122,31 @@ int eggman_autotune(uint64_t plan, int dtype, int64_t B, int reps, char *report,
             if (c >= 64 && std::find(chunks.begin(), chunks.end(), c) == chunks.end()) chunks.push_back(c);
         }
         for (int64_t c : chunks) {
-            for (int occ : {1, 2, 4}) cands.push_back(Tune{64, c, true, occ});
-            cands.push_back(Tune{64, c, false, 0});
+            for (int occ : {1, 2}) {
+                Tune t;
+                t.chunk = c;
+                t.occ = occ;
+                cands.push_back(t);
+                t.wide = true;
+                cands.push_back(t);
+            }
+            Tune g;
+            g.chunk = c;
+            g.lds = false;
+            cands.push_back(g);
         }
     } else {
-        const int bt = pow2_at_least(B);
-        cands.push_back(Tune{bt, B, false, 0});
-        if (bt != 64) cands.push_back(Tune{64, B, false, 0});
-        cands.push_back(Tune{64, B, true, 0});
+        Tune t;
+        t.bt = pow2_at_least(B);
+        t.chunk = B;
+        t.lds = false;
+        cands.push_back(t);
+        if (t.bt != 64) {
+            t.bt = 64;
+            cands.push_back(t);
+        }
+        t.lds = true;
+        cands.push_back(t);
     }
     if (fused_lds_bytes(pl->lay, dtype) <= ds->caps.max_lds_block) {
         Tune f;
@@ -1147,8 +1184,8 @@ int eggman_autotune(uint64_t plan, int dtype, int64_t B, int reps, char *report,
         }
         ms /= reps;
         char line[160];
-        snprintf(line, sizeof line, "bt=%d chunk=%lld lds=%d occ=%d graph=%d fused=%d ms=%.4f\n",
-                 t.bt, (long long)t.chunk, (int)t.lds, t.occ, (int)t.graph, (int)t.fused, ms);
+        snprintf(line, sizeof line, "bt=%d chunk=%lld flags=%d occ=%d ms=%.4f\n", t.bt,
+                 (long long)t.chunk, tune_flags(t), t.occ, ms);
         rep += line;
         if (ms < best) {
             best = ms;
@@ -1169,9 +1206,8 @@ int eggman_autotune(uint64_t plan, int dtype, int64_t B, int reps, char *report,
     if (rc != EGGMAN_OK) return rc;
     g_tune[TuneKey(pl->device, pl->lay.K, pl->lay.index_bytes, dtype, batch_bucket(B))] = best_t;
     char line[160];
-    snprintf(line, sizeof line, "best bt=%d chunk=%lld lds=%d occ=%d graph=%d fused=%d ms=%.4f\n",
-             best_t.bt, (long long)best_t.chunk, (int)best_t.lds, best_t.occ, (int)best_t.graph,
-             (int)best_t.fused, best);
+    snprintf(line, sizeof line, "best bt=%d chunk=%lld flags=%d occ=%d ms=%.4f\n", best_t.bt,
+             (long long)best_t.chunk, tune_flags(best_t), best_t.occ, best);
     rep += line;
     if (report && report_len > 0) {
         const size_t n = std::min<size_t>(rep.size(), (size_t)report_len - 1);
@@ -1181,20 +1217,28 @@ int eggman_autotune(uint64_t plan, int dtype, int64_t B, int reps, char *report,
     return EGGMAN_OK;
 }
 
-int eggman_tune_set(int K, int index_bytes, int dtype, int64_t B, int bt, int64_t chunk, int lds,
-                    int occ, int graph, int fused) {
+int eggman_tune_set(int K, int index_bytes, int dtype, int64_t B, int bt, int64_t chunk,
+                    int flags, int occ) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
-    if (!elt_hbytes(dtype) || B < 1 || (bt & (bt - 1)) || bt < 1 || bt > 64 || occ < 0)
+    if (!elt_hbytes(dtype) || B < 1 || (bt & (bt - 1)) || bt < 1 || bt > 64 || occ < 0 ||
+        (flags & ~(TUNE_LDS | TUNE_GRAPH | TUNE_FUSED | TUNE_WIDE)))
         return fail(EGGMAN_EINVAL, "bad tuning entry");
     int dev;
     EGG_TRY(current_device(&dev));
-    g_tune[TuneKey(dev, K, index_bytes, dtype, batch_bucket(B))] =
-        Tune{bt, chunk, lds != 0, occ, graph != 0, fused != 0};
+    Tune t;
+    t.bt = bt;
+    t.chunk = chunk;
+    t.occ = occ;
+    t.lds = (flags & TUNE_LDS) != 0;
+    t.graph = (flags & TUNE_GRAPH) != 0;
+    t.fused = (flags & TUNE_FUSED) != 0;
+    t.wide = (flags & TUNE_WIDE) != 0;
+    g_tune[TuneKey(dev, K, index_bytes, dtype, batch_bucket(B))] = t;
     return EGGMAN_OK;
 }
 
 int eggman_tune_get(int K, int index_bytes, int dtype, int64_t B, int *bt, int64_t *chunk,
-                    int *lds, int *occ, int *graph, int *fused) {
+                    int *flags, int *occ) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     int dev;
     EGG_TRY(current_device(&dev));
@@ -1202,10 +1246,8 @@ int eggman_tune_get(int K, int index_bytes, int dtype, int64_t B, int *bt, int64
     if (it == g_tune.end()) return fail(EGGMAN_EINVAL, "no tuning entry");
     if (bt) *bt = it->second.bt;
     if (chunk) *chunk = it->second.chunk;
-    if (lds) *lds = it->second.lds;
+    if (flags) *flags = tune_flags(it->second);
     if (occ) *occ = it->second.occ;
-    if (graph) *graph = it->second.graph;
-    if (fused) *fused = it->second.fused;
     return EGGMAN_OK;
 }
 
@@ -1321,8 +1363,8 @@ int eggman_dp_forward_grad(uint64_t plan, int dtype, const void *P_host, int64_t
     EGG_TRY(ensure(ds->P, (size_t)chunk * npairs * pb));
     EGG_TRY(ensure(ds->dP, (size_t)chunk * npairs * hb));
     EGG_TRY(ensure(ds->out, (size_t)chunk * hb));
-    Tune tune = default_tune(pl->lay, dtype, chunk);
-    tune.chunk = chunk;
+    // the forward's tuned choices (instances per pass, waves per workgroup) serve the gradient
+    const Tune tune = pick_tune(*pl, dtype, chunk);
     std::vector<char> stage, gstage;
     for (int64_t b0 = 0; b0 < B; b0 += chunk) {
         const int64_t nb = std::min(chunk, B - b0);
@@ -1378,8 +1420,7 @@ int eggman_dp_forward_grad_device(uint64_t plan, int dtype, const void *P_dev, i
     EGG_TRY(ensure_rev(*pl));
     int64_t chunk = B;
     EGG_TRY(grad_chunk(*pl, dtype, B, &chunk));
-    Tune tune = default_tune(pl->lay, dtype, chunk);
-    tune.chunk = chunk;
+    const Tune tune = pick_tune(*pl, dtype, chunk);
     return forward_grad_any(dtype, *ds, *pl, P_dev, ldp, B, out_dev, dP_dev, lddp, st, tune);
 }
 

@@ -61,8 +61,23 @@ __global__ __launch_bounds__(kBlock) void k_level_fwd(
         if (b >= nb) continue;
         const W *tw = trans_lvl + s * (int64_t)k;
         typename E::H acc = E::zero();
-#pragma unroll 4
-        for (int i = 0; i < k; i++) {
+        constexpr int U = 8;   // loads issued together before they are summed in order
+        int i = 0;
+        for (; i + U <= k; i += U) {
+            W t[U];
+            typename E::P pv[U];
+            typename E::H hv[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) t[u] = tw[i + u];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                pv[u] = P[((int64_t)(t[u] & mask) - 1) * ldp + b];
+                hv[u] = Hc[((int64_t)(t[u] >> bits) - cfirst) * ldh + b];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) acc = E::muladd(acc, pv[u], hv[u]);
+        }
+        for (; i < k; i++) {
             const W t = tw[i];
             const int64_t q = (int64_t)(t & mask) - 1;
             const int64_t c = (int64_t)(t >> bits) - cfirst;
@@ -150,14 +165,27 @@ __device__ inline void level_lds_body(const LevelArgs<DT, W> &a, int64_t j,
     __syncthreads();
 
     const int64_t step = bstride * WPB;
+    constexpr int U = 8;   // child loads issued together before they are summed in order
     for (int64_t s0 = s_begin + bi * WPB; s0 < s_end; s0 += step) {
         const int64_t s = wave_uniform(s0 + wave);
         if (s >= s_end) break;
         const W *tw = a.trans_lvl + s * (int64_t)a.k;
         typename E::H acc = E::zero();
         if (live) {
-#pragma unroll 4
-            for (int i = 0; i < a.k; i++) {
+            int i = 0;
+            for (; i + U <= a.k; i += U) {
+                W t[U];
+                typename E::H h[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) t[u] = tw[i + u];
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    h[u] = a.Hc[((int64_t)(t[u] >> bits) - a.cfirst) * a.ldh + b];
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    acc = E::muladd(acc, lds[((int)(t[u] & mask) - 1 - a.qlo) * 64 + lane], h[u]);
+            }
+            for (; i < a.k; i++) {
                 const W t = tw[i];
                 const int q = (int)(t & mask) - 1 - a.qlo;
                 const int64_t c = (int64_t)(t >> bits) - a.cfirst;
@@ -168,7 +196,7 @@ __device__ inline void level_lds_body(const LevelArgs<DT, W> &a, int64_t j,
     }
 }
 
-template <int DT, typename W, int WPB>
+template <int DT, typename W, int WPB, int TAG>
 __global__ __launch_bounds__(WPB * 64) void k_level_fwd_lds(LevelArgs<DT, W> a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     level_lds_body<DT, W, WPB>(a, (int64_t)blockIdx.x,

@@ -81,8 +81,23 @@ __global__ __launch_bounds__(WPB * 64) void k_rev_g(
         const int64_t e0 = coff[c], e1 = coff[c + 1];
         typename E::H acc = E::zero();
         if (live) {
-#pragma unroll 4
-            for (int64_t e = e0; e < e1; e++) {
+            constexpr int U = 8;   // gathers issued together before they are summed in order
+            int64_t e = e0;
+            for (; e + U <= e1; e += U) {
+                W t[U];
+                typename E::H g[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) t[u] = cent[e + u];
+#pragma unroll
+                for (int u = 0; u < U; u++) g[u] = Gs[(int64_t)(t[u] >> bits) * ldg + b];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const int q = (int)(t[u] & mask) - 1 - qlo;
+                    const PT pv = USE_LDS ? lds[q * 64 + lane] : P[(int64_t)(qlo + q) * ldp + b];
+                    acc = E::muladd(acc, pv, g[u]);
+                }
+            }
+            for (; e < e1; e++) {
                 const W t = cent[e];
                 const int q = (int)(t & mask) - 1 - qlo;
                 const int64_t s = (int64_t)(t >> bits);
@@ -113,10 +128,23 @@ __global__ __launch_bounds__(256) void k_rev_dp_seg(
         const int64_t b = tile * 64 + lane;
         if (b >= nb) continue;
         const RevSegDev sg = segs[seg0 + si];
+        const uint64_t *pe = pent + sg.start;
         typename E::H acc = E::zero();
-#pragma unroll 4
-        for (int i = 0; i < sg.len; i++) {
-            const uint64_t w = pent[sg.start + i];
+        constexpr int U = 8;   // gathers issued together before they are summed in order
+        int i = 0;
+        for (; i + U <= sg.len; i += U) {
+            typename E::H hv[U], gv[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const uint64_t w = pe[i + u];
+                hv[u] = Hc[(int64_t)(w & 0xffffffffull) * ldh + b];
+                gv[u] = Gs[(int64_t)(w >> 32) * ldg + b];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) acc = E::muladd_hh(acc, hv[u], gv[u]);
+        }
+        for (; i < sg.len; i++) {
+            const uint64_t w = pe[i];
             const int64_t s = (int64_t)(w >> 32), c = (int64_t)(w & 0xffffffffull);
             acc = E::muladd_hh(acc, Hc[c * ldh + b], Gs[s * ldg + b]);
         }

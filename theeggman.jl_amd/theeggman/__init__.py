@@ -21,6 +21,18 @@ from .tuning import autotune, load_tuning, save_tuning, tune_get, tune_set
 
 _load_lib()
 
+
+def _load_default_tuning():
+    """Launch choices measured on an MI355X ship beside the package; load them when a GPU is
+    visible (every choice gives the same results, so this only affects speed)."""
+    import os
+    from .tuning import DEFAULT_TABLE
+    if os.path.exists(DEFAULT_TABLE) and _load_lib().eggman_device_count() > 0:
+        load_tuning(DEFAULT_TABLE)
+
+
+_load_default_tuning()
+
 __all__ = ["hafnian", "hafnian_repeated", "hafnian_exact", "hafnian_grad", "hafnian_mixed",
            "hafnian_repeated_grad", "rrule", "reduction", "HIPBackend",
            "gpu_cache_bytes", "empty_gpu_cache", "dp_batch_bytes", "DP_MAX", "DP_HARD_MAX",

@@ -56,10 +56,9 @@ SIGNATURES = {
     "eggman_dp_forward_grad_device": (_int, [ctypes.c_uint64, _int, _vp, _i64, _i64, _vp, _vp,
                                              _i64, _vp]),
     "eggman_autotune": (_int, [ctypes.c_uint64, _int, _i64, _int, ctypes.c_char_p, _i64]),
-    "eggman_tune_set": (_int, [_int, _int, _int, _i64, _int, _i64, _int, _int, _int, _int]),
+    "eggman_tune_set": (_int, [_int, _int, _int, _i64, _int, _i64, _int, _int]),
     "eggman_tune_get": (_int, [_int, _int, _int, _i64, ctypes.POINTER(_int), _i64p,
-                               ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(_int),
-                               ctypes.POINTER(_int)]),
+                               ctypes.POINTER(_int), ctypes.POINTER(_int)]),
     "eggman_cache_bytes": (_i64, []),
     "eggman_cache_clear": (_int, []),
     "eggman_batch_bytes": (_i64, [_int, _int, _i64]),

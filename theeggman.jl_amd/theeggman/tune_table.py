@@ -1,0 +1,54 @@
+"""Regenerate the launch-choice table for this GPU.
+
+    python theeggman.jl_amd/theeggman/tune_table.py [--out PATH] [--quick]
+
+Runs the autotuner over the degrees, element types and batch sizes the benchmarks and tests use
+and writes the table (JSON) that ``theeggman`` loads at import (``tuning_mi355x.json`` beside
+this file by default).  Every launch choice gives bit-identical results; the table only changes
+speed.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=os.path.join(HERE, "tuning_mi355x.json"))
+    ap.add_argument("--quick", action="store_true", help="only the N=28 benchmark shapes")
+    args = ap.parse_args()
+    import numpy as np
+    import theeggman as te
+
+    shapes = [(28, np.float64, 1024), (28, np.float64, 1), (28, np.complex128, 1)]
+    if not args.quick:
+        for K in (16, 20, 24, 28, 32):
+            for B in (1, 64, 256, 1024, 4096):
+                shapes.append((K, np.float64, B))
+        for K in (20, 24, 28, 32):
+            shapes.append((K, np.complex128, 1))
+            shapes.append((K, np.complex128, 256))
+        for K in (36, 40):
+            shapes.append((K, np.complex128, 1))
+        shapes.append((40, np.int64, 64))
+    seen = set()
+    for K, T, B in shapes:
+        if (K, np.dtype(T).str, B) in seen:
+            continue
+        seen.add((K, np.dtype(T).str, B))
+        backend = te.HIPBackend(plan_source="generate") if K > 32 else te.HIPBackend()
+        r = te.autotune(K, T, B, reps=3 if B <= 1024 else 2, backend=backend)
+        c = r["choice"]
+        print(f"K={K} {np.dtype(T).name} B={B}: bt={c['bt']} chunk={c['chunk']} flags={c['flags']} "
+              f"occ={c['occ']} {c['ms']:.4f} ms", flush=True)
+    te.save_tuning(args.out)
+    print("wrote", args.out, flush=True)
+
+
+if __name__ == "__main__":
+    main()

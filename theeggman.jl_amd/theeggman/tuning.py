@@ -1,15 +1,18 @@
 """Autotuner front end: choose launch parameters per (degree, element type, batch size).
 
 The library times its candidate launch shapes -- lanes per state, LDS staging of the pair rows,
-workgroups per CU, instances per pass over the levels, and eager per-level launches versus a
-replayed HIP graph of the whole level sequence -- and keeps the fastest.  Every candidate computes
-bit-identical results, so tuning only ever changes speed.  Tables can be saved to and loaded from
-a JSON file so a tuned choice survives the process.
+8 or 16 waves per workgroup, workgroups per CU, instances per pass over the levels, one fused
+LDS-resident kernel for small degrees, and eager per-level launches versus a replayed HIP graph
+of the whole level sequence -- and keeps the fastest.  Every candidate computes bit-identical
+results, so tuning only ever changes speed.  Tables can be saved to and loaded from a JSON file
+so a tuned choice survives the process; the package loads ``tuning_mi355x.json`` beside it at
+import when present.
 """
 from __future__ import annotations
 
 import ctypes
 import json
+import os
 
 import numpy as np
 
@@ -17,13 +20,18 @@ from ._lib import check, lib
 from .backend import HIPBackend, _device_plan, dtype_code
 from .plan import DP_MAX, PlanRef, _dp_plan
 
-_FIELDS = ("bt", "chunk", "lds", "occ", "graph", "fused")
-_TUNED: dict = {}   # (K, index_bytes, dtype code, B) -> choice, for save_tuning
+LDS, GRAPH, FUSED, WIDE = 1, 2, 4, 8        # EGGMAN_TUNE_* flags
+_NAMES = {"lds": LDS, "graph": GRAPH, "fused": FUSED, "wide": WIDE}
+DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_mi355x.json")
+_TUNED: dict = {}   # (K, index_bytes, dtype code, B) -> {"bt", "chunk", "flags", "occ"}
 
 
 def _parse(line: str) -> dict:
     kv = dict(tok.split("=") for tok in line.split() if "=" in tok)
-    return {k: (float(v) if k == "ms" else int(v)) for k, v in kv.items()}
+    d = {k: (float(v) if k == "ms" else int(v)) for k, v in kv.items()}
+    for name, bit in _NAMES.items():
+        d[name] = int(bool(d.get("flags", 0) & bit))
+    return d
 
 
 def autotune(K: int, dtype=np.float64, B: int = 1024, reps: int = 3,
@@ -40,31 +48,32 @@ def autotune(K: int, dtype=np.float64, B: int = 1024, reps: int = 3,
     lines = buf.value.decode().strip().splitlines()
     cands = [_parse(ln) for ln in lines if not ln.startswith("best")]
     choice = _parse(lines[-1])
-    _TUNED[(K, np.dtype(I).itemsize, code, B)] = {k: choice[k] for k in _FIELDS}
+    _TUNED[(K, np.dtype(I).itemsize, code, B)] = {k: choice[k] for k in ("bt", "chunk", "flags", "occ")}
     return {"choice": choice, "candidates": cands}
 
 
 def tune_get(K: int, dtype=np.float64, B: int = 1024):
     """The tuning entry the library uses for (K, dtype, B), or None."""
     ib = 4 if K <= DP_MAX else 8
-    vals = [ctypes.c_int() for _ in range(5)]
+    bt, flags, occ = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     chunk = ctypes.c_int64()
-    rc = lib().eggman_tune_get(K, ib, dtype_code(dtype), B, ctypes.byref(vals[0]),
-                               ctypes.byref(chunk), ctypes.byref(vals[1]), ctypes.byref(vals[2]),
-                               ctypes.byref(vals[3]), ctypes.byref(vals[4]))
+    rc = lib().eggman_tune_get(K, ib, dtype_code(dtype), B, ctypes.byref(bt), ctypes.byref(chunk),
+                               ctypes.byref(flags), ctypes.byref(occ))
     if rc != 0:
         return None
-    return {"bt": vals[0].value, "chunk": chunk.value, "lds": vals[1].value,
-            "occ": vals[2].value, "graph": vals[3].value, "fused": vals[4].value}
+    d = {"bt": bt.value, "chunk": chunk.value, "flags": flags.value, "occ": occ.value}
+    for name, bit in _NAMES.items():
+        d[name] = int(bool(flags.value & bit))
+    return d
 
 
-def tune_set(K: int, dtype, B: int, bt: int, chunk: int, lds: int, occ: int, graph: int = 0,
-             fused: int = 0) -> None:
+def tune_set(K: int, dtype, B: int, bt: int = 64, chunk: int = 0, occ: int = 0, *,
+             lds: bool = True, graph: bool = False, fused: bool = False, wide: bool = False) -> None:
     ib = 4 if K <= DP_MAX else 8
     code = dtype_code(dtype)
-    check(lib().eggman_tune_set(K, ib, code, B, bt, chunk, lds, occ, graph, fused), "tune_set")
-    _TUNED[(K, ib, code, B)] = {"bt": bt, "chunk": chunk, "lds": lds, "occ": occ, "graph": graph,
-                                "fused": fused}
+    flags = (LDS if lds else 0) | (GRAPH if graph else 0) | (FUSED if fused else 0) | (WIDE if wide else 0)
+    check(lib().eggman_tune_set(K, ib, code, B, bt, chunk, flags, occ), "tune_set")
+    _TUNED[(K, ib, code, B)] = {"bt": bt, "chunk": chunk, "flags": flags, "occ": occ}
 
 
 def save_tuning(path: str) -> None:
@@ -74,12 +83,12 @@ def save_tuning(path: str) -> None:
         json.dump(rows, f, indent=1)
 
 
-def load_tuning(path: str) -> int:
+def load_tuning(path: str = DEFAULT_TABLE) -> int:
+    """Load a saved table into the library (for the current device).  Returns the entries read."""
     with open(path) as f:
         rows = json.load(f)
     for r in rows:
         check(lib().eggman_tune_set(r["K"], r["index_bytes"], r["dtype"], r["B"], r["bt"],
-                                    r["chunk"], r["lds"], r["occ"], r.get("graph", 0),
-                                    r.get("fused", 0)), "tune_set")
-        _TUNED[(r["K"], r["index_bytes"], r["dtype"], r["B"])] = {k: r.get(k, 0) for k in _FIELDS}
+                                    r["chunk"], r["flags"], r["occ"]), "tune_set")
+        _TUNED[(r["K"], r["index_bytes"], r["dtype"], r["B"])] = {k: r[k] for k in ("bt", "chunk", "flags", "occ")}
     return len(rows)
