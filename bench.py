@@ -94,9 +94,102 @@ def cpu_baseline(K, budget_s=12.0):
                       f"{nthreads} threads, min of {reps} reps ({spent:.1f} s)"}
 
 
+def run_configs(path):
+    """Measure the five workload configurations of the path (device times with inputs resident
+    in HBM; the CPU restatement beside the small ones) and write them to `path` as JSON."""
+    import numpy as np
+    import theeggman as te
+    from theeggman import _lib, hip, synth
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    st = hip.Stream()
+    res = {}
+
+    def dev_forward(As_or_P, K, T, steps, backend=None, P=None):
+        be = backend or te.HIPBackend()
+        plan = te.PlanRef(K, 8) if be.plan_source == "generate" else te._dp_plan(K)
+        if P is None:
+            P = synth.pair_batch(np.asarray(As_or_P)).astype(T)
+        B = P.shape[1]
+        Pd = hip.DeviceBuffer.from_array(P)
+        out = hip.DeviceBuffer(B * 16)
+        fn = lambda: te._haf_dp_device(be, plan, T, Pd.addr, B, B, out.addr, st.handle)
+        ms, _ = time_device(hip, st, fn, steps, 2)
+        return ms, B
+
+    # 1: plain N=20 real Float64, Uniform(-1, 1); the reference runs it on the CPU as-is
+    A = synth.sym_uniform(20, 1, 0)
+    ms, _ = dev_forward(A[None], 20, np.float64, 200)
+    P = synth.pair_batch(A[None])
+    oracle.build_plan(20)
+    t0 = time.perf_counter()
+    for _ in range(20):
+        oracle.haf_dp(P, 20, oracle.F64, 1)
+    cpu_ms = (time.perf_counter() - t0) / 20 * 1e3
+    res["1_plain_n20_f64"] = {"gpu_ms": ms, "cpu_restatement_1thread_ms": cpu_ms,
+                              "tuned": te.tune_get(20, np.float64, 1)}
+    # 2: plain N=28 ComplexF64, (g1 + i g2)/sqrt(2N)
+    ms, _ = dev_forward(synth.sym_complex_normal(28, 2, 0)[None], 28, np.complex128, 100)
+    res["2_plain_n28_c64"] = {"gpu_ms": ms, "hafnians_per_s": 1e3 / ms}
+    # 3a: 4096 real N=24 matrices, N(0, 1/N)
+    As = np.stack([synth.sym_normal(24, 3, b, 1 / np.sqrt(24)) for b in range(4096)])
+    ms, B = dev_forward(As, 24, np.float64, 10)
+    _, nt = te._dp_counts(24)
+    res["3a_batched_n24_b4096_f64"] = {"gpu_ms": ms, "hafnians_per_s": B / ms * 1e3,
+                                       "transitions_per_s": B * nt / ms * 1e3}
+    # 3b: 2048 matrices, degrees uniform on {16, 20, 24, 28, 32}, one pass
+    rng = np.random.default_rng(3)
+    degs = rng.choice([16, 20, 24, 28, 32], size=2048)
+    groups = {K: np.stack([synth.sym_normal(int(K), 5, i, 1 / np.sqrt(K))
+                           for i in np.flatnonzero(degs == K)]) for K in (16, 20, 24, 28, 32)}
+    Ps = [synth.pair_batch(groups[K]) for K in sorted(groups)]
+    plans = [te._dp_plan(K) for K in sorted(groups)]
+    hs = [te._device_plan(te.HIPBackend(), p) for p in plans]
+    flat = np.concatenate([P.ravel() for P in Ps])
+    Pd = hip.DeviceBuffer.from_array(flat)
+    out = hip.DeviceBuffer(2048 * 8)
+    import ctypes
+    harr = (ctypes.c_uint64 * 5)(*hs)
+    carr = (ctypes.c_int64 * 5)(*[P.shape[1] for P in Ps])
+    fn = lambda: _lib.check(_lib.lib().eggman_dp_forward_mixed_device(
+        5, harr, carr, _lib.F64, Pd.addr, out.addr, st.handle), "mixed")
+    ms, _ = time_device(hip, st, fn, 5, 1)
+    ntr = sum(P.shape[1] * te._dp_counts(K)[1] for P, K in zip(Ps, sorted(groups)))
+    res["3b_mixed_2048_k16_32_f64"] = {"gpu_ms": ms, "hafnians_per_s": 2048 / ms * 1e3,
+                                       "transitions_per_s": ntr / ms * 1e3,
+                                       "counts": {int(K): int(groups[K].shape[0]) for K in groups}}
+    # 4: 256 real N=24 matrices, Uniform(-1, 1), forward + full gradient
+    As = np.stack([synth.sym_uniform(24, 4, b) for b in range(256)])
+    P = synth.pair_batch(As)
+    Pd = hip.DeviceBuffer.from_array(P)
+    out = hip.DeviceBuffer(256 * 8)
+    dP = hip.DeviceBuffer(256 * 8 * P.shape[0])
+    h = te._device_plan(te.HIPBackend(), te._dp_plan(24))
+    fn = lambda: _lib.check(_lib.lib().eggman_dp_forward_grad_device(
+        h, _lib.F64, Pd.addr, 256, 256, out.addr, dP.addr, 256, st.handle), "grad")
+    ms, _ = time_device(hip, st, fn, 10, 2)
+    res["4_grad_n24_b256_f64"] = {"gpu_ms": ms, "hafnians_with_gradient_per_s": 256 / ms * 1e3}
+    # 5a: one ComplexF64 matrix at N=36 and N=40, Int64 plan generated in HBM
+    gen = te.HIPBackend(plan_source="generate")
+    for K in (36, 40):
+        ms, _ = dev_forward(synth.sym_complex_normal(K, 2, 0)[None], K, np.complex128, 5, backend=gen)
+        res[f"5a_plain_n{K}_c64"] = {"gpu_ms": ms, "transitions_per_s": te._dp_counts(K)[1] / ms * 1e3}
+    # 5b: 64 G(40, 1/2) graphs, exact perfect-matching counts on the 128-bit path
+    As = np.stack([synth.gnp_adjacency(40, 52, b) for b in range(64)])
+    ms, B = dev_forward(As, 40, np.int64, 2, backend=gen)
+    res["5b_exact_n40_b64"] = {"gpu_ms": ms, "hafnians_per_s": B / ms * 1e3,
+                               "transitions_per_s": B * te._dp_counts(40)[1] / ms * 1e3}
+    with open(path, "w") as f:
+        json.dump(res, f, indent=1)
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--configs", metavar="PATH",
+                    help="instead of the benchmark line, measure the five workload configurations "
+                         "and write them to PATH")
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=1024)
@@ -117,6 +210,9 @@ def main():
               "shards by instance with no communication, so its whole-job rate is N times the "
               "one-GPU value", file=sys.stderr)
         sys.exit(2)
+    if args.configs:
+        run_configs(args.configs)
+        return
 
     import numpy as np
     import theeggman as te

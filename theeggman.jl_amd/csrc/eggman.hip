@@ -666,7 +666,9 @@ static int mixed_plans(int ngroups, const uint64_t *plans, const int64_t *counts
 
 // ------------------------------------------------------------------------------ reverse sweep
 
-constexpr int kRevSegLen = 1024;   // transitions per dP segment
+// Transitions per dP segment: short enough that every level yields many (segment, tile) waves
+// to hide the latency of the two gathers per transition.
+constexpr int kRevSegLen = 256;
 
 static int upload_vec(void **dst, const void *src, size_t bytes, size_t *acc) {
     hipError_t e = hipMalloc(dst, std::max<size_t>(bytes, 16));
