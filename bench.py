@@ -291,6 +291,23 @@ def main():
                      "avg_launch_ms": ms / launches},
     }
 
+    rl = result["roofline"]
+    if args.copy_calibration <= 0:
+        # measured device-copy bandwidth on this GPU (reads + writes of a 2 GiB copy), the
+        # practical ceiling beside the nominal 8 TB/s (skipped in counter runs, whose one copy
+        # must be the calibration copy)
+        cbytes = 2 << 30
+        csrc, cdst = hip.DeviceBuffer(cbytes), hip.DeviceBuffer(cbytes)
+        csrc.zero()
+        cms, _ = time_device(hip, stream, lambda: hip.copy_d2d_async(cdst, csrc, cbytes, stream), 5, 1)
+        copy_gbs = 2 * cbytes / (cms / 1e3) / 1e9
+        del csrc, cdst
+        rl["measured_copy_gbs"] = copy_gbs
+        rl["frac_of_copy"] = rl["achieved"] / copy_gbs
+    if rl["traffic"]:
+        rl["traffic_gbs"] = rl["traffic"] / (rl["avg_launch_ms"] / 1e3) / 1e9
+        rl["traffic_frac"] = rl["traffic_gbs"] / HBM_PEAK_GBS
+
     if args.copy_calibration > 0:
         nbytes_copy = args.copy_calibration << 20
         src, dst = hip.DeviceBuffer(nbytes_copy), hip.DeviceBuffer(nbytes_copy)

@@ -183,8 +183,10 @@ int eggman_dp_forward_grad_device(uint64_t plan, int dtype, const void *P_dev, i
  * Time the launch choices for (plan, dtype, B) -- lanes per state, LDS staging of the pair rows,
  * workgroups per CU, and the instances per pass over the levels -- on synthetic pair values,
  * `reps` runs each, and keep the fastest for this (device, degree, layout, dtype, B rounded up
- * to a power of two).  Later calls of the forward entry points use it.  Every choice gives
- * bit-identical results.  A text report (one line per candidate) is written to `report`.
+ * to a power of two).  Later calls of the forward entry points use it; a batch size with no entry
+ * of its own uses the nearest tuned bucket of the same degree, layout and dtype (the largest
+ * below it, else the smallest above).  Every choice gives bit-identical results.  A text report
+ * (one line per candidate) is written to `report`.
  * New: the reference has no tuning.
  */
 int eggman_autotune(uint64_t plan, int dtype, int64_t B, int reps, char *report,

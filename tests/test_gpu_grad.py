@@ -150,3 +150,37 @@ def test_rrule_pullback():
     assert y == te.hafnian(A)
     none, dA = pb(2.5)
     assert none is None and np.array_equal(dA, 2.5 * G)
+
+
+@pytest.mark.parametrize("N,B", [(28, 4), (32, 1)])
+def test_exact_gradient_large_graphs_sampled_deletion(N, B):
+    # G(N, 1/2) adjacency matrices on the 128-bit path: sampled entries equal the deletion
+    # hafnian exactly, and the full gradient satisfies homogeneity exactly
+    note(f"exact grad graphs N={N} B={B}")
+    As = np.stack([synth.gnp_adjacency(N, 61, b) for b in range(B)]).astype(np.int64)
+    P = te._gather_pairs_batch(np.int64, As, None, N)
+    vals, dP = te._haf_dp_backend_grad(BE, P, te._dp_plan(N))
+    want = o.haf_dp(synth.pair_batch(As).astype(np.int64), N, o.I128, 8)
+    pairs = [(0, 1), (0, N - 1), (3, N // 2), (N // 2 - 1, N // 2), (N - 2, N - 1)]
+    for b in range(B):
+        assert vals[b] == want[b]
+        for (i, j) in pairs[: 5 if N <= 28 else 2]:
+            assert dP[b][te._pair_index(i + 1, j + 1, N) - 1] == deletion(As[b], i, j, o.I128), (b, i, j)
+        hom = sum(int(As[b][i, j]) * dP[b][te._pair_index(i + 1, j + 1, N) - 1]
+                  for i in range(N) for j in range(i + 1, N))
+        assert hom * 2 == N * vals[b]
+
+
+def test_float_gradient_n28_sampled_deletion():
+    note("grad f64 N=28 B=64")
+    N, B = 28, 64
+    As = np.stack([synth.sym_normal(N, 62, b, 1 / np.sqrt(N)) for b in range(B)])
+    vals, Gs = te.hafnian_grad(As, check_symmetric=False)
+    assert vals.tobytes() == te.hafnian(As, check_symmetric=False).tobytes()
+    for b in (0, 37, 63):
+        scale = np.abs(Gs[b]).max()
+        for (i, j) in [(0, 1), (0, 27), (9, 20), (13, 14), (26, 27)]:
+            d = deletion(As[b], i, j, o.F64)
+            assert abs(Gs[b, i, j] - d) <= 1e-12 * scale, (b, i, j)
+        lhs = np.sum(np.triu(As[b], 1) * np.triu(Gs[b], 1))
+        assert abs(lhs - N / 2 * vals[b]) <= 1e-11 * max(1.0, np.abs(As[b]).sum() * scale)

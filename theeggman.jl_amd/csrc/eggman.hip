@@ -51,6 +51,12 @@ struct DevPlan {
     void *d_qoff = nullptr, *d_qseg = nullptr;
     std::vector<int64_t> lvl_seg;
     int64_t nseg = 0;
+    // parent-major dP tasks (used when the plan uses the Int32 layout)
+    void *d_prank = nullptr, *d_tasks = nullptr, *d_roff = nullptr, *d_rtask = nullptr;
+    void *d_qrow = nullptr, *d_qcol = nullptr;
+    void *d_red = nullptr, *d_srow0 = nullptr, *d_sruns = nullptr;
+    std::vector<int64_t> lvl_task;
+    int64_t part_rows = 0, nred = 0;
     size_t rev_bytes = 0;
 };
 
@@ -238,9 +244,22 @@ static int64_t batch_bucket(int64_t B) {
     return b;
 }
 
+// The entry for B's bucket; else the nearest tuned bucket of the same degree and element type
+// (the largest one below B, or failing that the smallest above); else the defaults.
 static Tune pick_tune(const DevPlan &pl, int dtype, int64_t B) {
-    auto it = g_tune.find(TuneKey(pl.device, pl.lay.K, pl.lay.index_bytes, dtype, batch_bucket(B)));
-    if (it == g_tune.end()) return default_tune(pl.lay, dtype, B);
+    const TuneKey key(pl.device, pl.lay.K, pl.lay.index_bytes, dtype, batch_bucket(B));
+    auto it = g_tune.find(key);
+    if (it == g_tune.end()) {
+        auto hi = g_tune.lower_bound(key);
+        auto same = [&](std::map<TuneKey, Tune>::iterator i) {
+            return i != g_tune.end() && std::get<0>(i->first) == pl.device &&
+                   std::get<1>(i->first) == pl.lay.K && std::get<2>(i->first) == pl.lay.index_bytes &&
+                   std::get<3>(i->first) == dtype;
+        };
+        if (hi != g_tune.begin() && same(std::prev(hi))) it = std::prev(hi);
+        else if (same(hi)) it = hi;
+        else return default_tune(pl.lay, dtype, B);
+    }
     Tune t = it->second;
     if (t.bt > pow2_at_least(B) && t.bt != 64) t.bt = pow2_at_least(B);
     if (t.chunk <= 0 || t.chunk > B) t.chunk = B;
@@ -530,115 +549,139 @@ static int forward_any(int dtype, DevState &ds, const DevPlan &pl, const void *P
 // Groups of instances with different degrees (and so different plans) walked together: launch t
 // advances every group by one level (level t + 2 of each group that still has levels left), so
 // one launch carries the work of all degrees and small levels of one degree ride along with large
-// levels of another.  Each group keeps its own two-level state buffers; its last level writes its
-// results.  P is the concatenation of the groups' [pair][count] blocks, out of their results.
-template <int DT, typename W>
+// levels of another.  Each group uses its own tuned pass size (instances per pass, as the
+// single-degree walk does) and keeps two-level state buffers of that size; the groups' passes run
+// in rounds, round r walking pass r of every group that has one.  P is the concatenation of the
+// groups' [pair][count] blocks, out of their results.
+template <int DT, typename W, int WPB>
 static int forward_mixed_levels(DevState &ds, const std::vector<DevPlan *> &pls,
                                 const std::vector<int64_t> &counts, const void *P, void *out,
-                                hipStream_t st, const Tune &tune) {
+                                hipStream_t st) {
     using E = Elt<DT>;
     using HT = typename E::H;
     using PT = typename E::P;
     const size_t ng = pls.size();
     std::vector<const PT *> Pg(ng);
     std::vector<HT *> outg(ng), buf0(ng), buf1(ng);
+    std::vector<Tune> tg(ng);
+    std::vector<int64_t> chunk(ng);
     size_t poff = 0, ooff = 0, hoff = 0;
+    int64_t rounds = 0;
     for (size_t g = 0; g < ng; g++) {
         Pg[g] = (const PT *)P + poff;
         outg[g] = (HT *)out + ooff;
         poff += (size_t)pls[g]->lay.npairs * counts[g];
         ooff += (size_t)counts[g];
-        hoff += (size_t)2 * pls[g]->lay.max_level * counts[g];
+        tg[g] = pick_tune(*pls[g], DT, std::max<int64_t>(counts[g], 1));
+        chunk[g] = std::max<int64_t>(1, std::min<int64_t>(tg[g].chunk > 0 ? tg[g].chunk : counts[g], counts[g]));
+        hoff += (size_t)2 * pls[g]->lay.max_level * chunk[g];
+        if (counts[g] > 0) rounds = std::max<int64_t>(rounds, (counts[g] + chunk[g] - 1) / chunk[g]);
     }
     EGG_TRY(ensure(ds.H, hoff * sizeof(HT)));
     hoff = 0;
-    int steps = 0;
     for (size_t g = 0; g < ng; g++) {
-        const size_t rows = (size_t)pls[g]->lay.max_level * counts[g];
+        const size_t rows = (size_t)pls[g]->lay.max_level * chunk[g];
         buf0[g] = (HT *)ds.H.p + hoff;
         buf1[g] = buf0[g] + rows;
         hoff += 2 * rows;
-        steps = std::max(steps, pls[g]->lay.K / 2);
-        if (counts[g] > 0)
-            k_fill_one<DT><<<dim3((unsigned)((counts[g] + 255) / 256)), dim3(256), 0, st>>>(buf1[g], counts[g]);
     }
-    auto kern = k_level_fwd_lds_multi<DT, W, kLdsWaves>;
+    auto kern = k_level_fwd_lds_multi<DT, W, WPB>;
     static bool attr_set = false;
     if (!attr_set) {
         attr_set = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)ds.caps.max_lds_block) == hipSuccess;
         (void)hipGetLastError();
     }
-    for (int t = 0; t < steps; t++) {
-        const int L = t + 2;
-        MultiLevelArgs<DT, W> m;
-        m.n = 0;
-        m.first[0] = 0;
-        size_t lds_max = 0;
-        auto flush = [&]() {
-            if (m.n == 0) return;
-            kern<<<dim3((unsigned)m.first[m.n]), dim3(kLdsWaves * 64), lds_max, st>>>(m);
-            m.n = 0;
-            lds_max = 0;
-        };
+    for (int64_t r = 0; r < rounds; r++) {
+        std::vector<int64_t> b0(ng), nbg(ng, 0);
+        int steps = 0;
         for (size_t g = 0; g < ng; g++) {
-            const DevPlan &pl = *pls[g];
-            const PlanLayout &lay = pl.lay;
-            const int64_t nb = counts[g];
-            if (nb == 0 || L > lay.K / 2 + 1) continue;
-            HT *Hc = (L - 1) & 1 ? buf1[g] : buf0[g];
-            HT *Ho = L == lay.K / 2 + 1 ? outg[g] : (L & 1 ? buf1[g] : buf0[g]);
-            int qlo, nq;
-            level_pairs(lay, L, &qlo, &nq);
-            const size_t lds = (size_t)nq * 64 * sizeof(PT);
-            if (!attr_set || !tune.lds || lds > ds.caps.max_lds_block) {
-                launch_level_bt<DT, W>(pow2_at_least(nb), pl, L, Pg[g], nb, Hc, nb, Ho, nb, nb, st);
-                continue;
-            }
-            LevelArgs<DT, W> &a = m.g[m.n];
-            a.trans_lvl = (const W *)pl.d_trans + lay.tstart[L - 1];
-            a.nst = lay.nst[L - 1];
-            a.k = lay.size[L - 1] - 1;
-            a.qlo = qlo;
-            a.nq = nq;
-            a.P = Pg[g];
-            a.ldp = nb;
-            a.Hc = Hc;
-            a.cfirst = lay.levels[L - 2];
-            a.ldh = nb;
-            a.Ho = Ho;
-            a.ldo = nb;
-            a.nb = nb;
-            a.map = make_lds_map(ds.caps, tune.occ, a.nst, nb, lds);
-            m.first[m.n + 1] = m.first[m.n] + 8 * a.map.per_xcd;
-            m.n++;
-            lds_max = std::max(lds_max, lds);
-            if (m.n == kMaxGroups) flush();
+            b0[g] = r * chunk[g];
+            if (b0[g] >= counts[g]) continue;
+            nbg[g] = std::min(chunk[g], counts[g] - b0[g]);
+            steps = std::max(steps, pls[g]->lay.K / 2);
+            k_fill_one<DT><<<dim3((unsigned)((nbg[g] + 255) / 256)), dim3(256), 0, st>>>(buf1[g], nbg[g]);
         }
-        flush();
+        for (int t = 0; t < steps; t++) {
+            const int L = t + 2;
+            MultiLevelArgs<DT, W> m;
+            m.n = 0;
+            m.first[0] = 0;
+            size_t lds_max = 0;
+            auto flush = [&]() {
+                if (m.n == 0) return;
+                kern<<<dim3((unsigned)m.first[m.n]), dim3(WPB * 64), lds_max, st>>>(m);
+                m.n = 0;
+                lds_max = 0;
+            };
+            for (size_t g = 0; g < ng; g++) {
+                const DevPlan &pl = *pls[g];
+                const PlanLayout &lay = pl.lay;
+                const int64_t nb = nbg[g];
+                if (nb == 0 || L > lay.K / 2 + 1) continue;
+                const PT *Pc = Pg[g] + b0[g];
+                HT *Hc = (L - 1) & 1 ? buf1[g] : buf0[g];
+                HT *Ho = L == lay.K / 2 + 1 ? outg[g] + b0[g] : (L & 1 ? buf1[g] : buf0[g]);
+                int qlo, nq;
+                level_pairs(lay, L, &qlo, &nq);
+                const size_t lds = (size_t)nq * 64 * sizeof(PT);
+                if (!attr_set || !tg[g].lds || lds > ds.caps.max_lds_block) {
+                    launch_level_bt<DT, W>(std::min(tg[g].bt, pow2_at_least(nb)), pl, L, Pc, counts[g],
+                                           Hc, nb, Ho, nb, nb, st);
+                    continue;
+                }
+                LevelArgs<DT, W> &a = m.g[m.n];
+                a.trans_lvl = (const W *)pl.d_trans + lay.tstart[L - 1];
+                a.nst = lay.nst[L - 1];
+                a.k = lay.size[L - 1] - 1;
+                a.qlo = qlo;
+                a.nq = nq;
+                a.P = Pc;
+                a.ldp = counts[g];
+                a.Hc = Hc;
+                a.cfirst = lay.levels[L - 2];
+                a.ldh = nb;
+                a.Ho = Ho;
+                a.ldo = nb;
+                a.nb = nb;
+                a.map = make_lds_map(ds.caps, tg[g].occ, a.nst, nb, lds, WPB);
+                m.first[m.n + 1] = m.first[m.n] + 8 * a.map.per_xcd;
+                m.n++;
+                lds_max = std::max(lds_max, lds);
+                if (m.n == kMaxGroups) flush();
+            }
+            flush();
+        }
     }
     EGG_HIP(hipGetLastError());
     return EGGMAN_OK;
 }
 
+// The workgroup width follows the tuned choice of the group with the most transitions.
 template <int DT>
 static int forward_mixed_dt(DevState &ds, const std::vector<DevPlan *> &pls,
                             const std::vector<int64_t> &counts, const void *P, void *out,
-                            hipStream_t st, const Tune &tune) {
+                            hipStream_t st) {
+    size_t big = 0;
+    for (size_t g = 1; g < pls.size(); g++)
+        if ((double)pls[g]->lay.ntrans * counts[g] > (double)pls[big]->lay.ntrans * counts[big]) big = g;
+    const bool wide = pick_tune(*pls[big], DT, std::max<int64_t>(counts[big], 1)).wide;
     if (pls[0]->lay.index_bytes == 4)
-        return forward_mixed_levels<DT, int32_t>(ds, pls, counts, P, out, st, tune);
-    return forward_mixed_levels<DT, int64_t>(ds, pls, counts, P, out, st, tune);
+        return wide ? forward_mixed_levels<DT, int32_t, 16>(ds, pls, counts, P, out, st)
+                    : forward_mixed_levels<DT, int32_t, kLdsWaves>(ds, pls, counts, P, out, st);
+    return wide ? forward_mixed_levels<DT, int64_t, 16>(ds, pls, counts, P, out, st)
+                : forward_mixed_levels<DT, int64_t, kLdsWaves>(ds, pls, counts, P, out, st);
 }
 
 static int forward_mixed_any(int dtype, DevState &ds, const std::vector<DevPlan *> &pls,
                              const std::vector<int64_t> &counts, const void *P, void *out,
-                             hipStream_t st, const Tune &tune) {
+                             hipStream_t st) {
     switch (dtype) {
-    case EGGMAN_F64: return forward_mixed_dt<EGGMAN_F64>(ds, pls, counts, P, out, st, tune);
-    case EGGMAN_F32: return forward_mixed_dt<EGGMAN_F32>(ds, pls, counts, P, out, st, tune);
-    case EGGMAN_C64: return forward_mixed_dt<EGGMAN_C64>(ds, pls, counts, P, out, st, tune);
-    case EGGMAN_C32: return forward_mixed_dt<EGGMAN_C32>(ds, pls, counts, P, out, st, tune);
-    case EGGMAN_I128: return forward_mixed_dt<EGGMAN_I128>(ds, pls, counts, P, out, st, tune);
+    case EGGMAN_F64: return forward_mixed_dt<EGGMAN_F64>(ds, pls, counts, P, out, st);
+    case EGGMAN_F32: return forward_mixed_dt<EGGMAN_F32>(ds, pls, counts, P, out, st);
+    case EGGMAN_C64: return forward_mixed_dt<EGGMAN_C64>(ds, pls, counts, P, out, st);
+    case EGGMAN_C32: return forward_mixed_dt<EGGMAN_C32>(ds, pls, counts, P, out, st);
+    case EGGMAN_I128: return forward_mixed_dt<EGGMAN_I128>(ds, pls, counts, P, out, st);
     }
     return fail(EGGMAN_EINVAL, "unknown dtype %d", dtype);
 }
@@ -669,6 +712,12 @@ static int mixed_plans(int ngroups, const uint64_t *plans, const int64_t *counts
 // Transitions per dP segment: short enough that every level yields many (segment, tile) waves
 // to hide the latency of the two gathers per transition.
 constexpr int kRevSegLen = 256;
+// Parent-major dP: waves per workgroup, parents per wave of a task (so a task holds up to
+// kParWaves * kParSub parents), and tasks per run of the first reduction stage.
+constexpr int kParWaves = 8;
+constexpr int kParSub = 16;
+constexpr int kParTaskLen = kParWaves * kParSub;
+constexpr int kRedFanIn = 64;
 
 static int upload_vec(void **dst, const void *src, size_t bytes, size_t *acc) {
     hipError_t e = hipMalloc(dst, std::max<size_t>(bytes, 16));
@@ -691,7 +740,7 @@ static int ensure_rev(DevPlan &pl) {
     EGG_HIP(hipDeviceSynchronize());
     EGG_HIP(hipMemcpy(trans.data(), pl.d_trans, trans.size(), hipMemcpyDeviceToHost));
     RevPlanHost rp;
-    EGG_TRY(build_rev_plan(lay, trans.data(), kRevSegLen, &rp));
+    EGG_TRY(build_rev_plan(lay, trans.data(), kRevSegLen, kParTaskLen, kRedFanIn, &rp));
     std::vector<RevSegDev> segs(rp.segs.size());
     for (size_t i = 0; i < segs.size(); i++)
         segs[i] = RevSegDev{rp.segs[i].start, rp.segs[i].len, rp.segs[i].level};
@@ -702,6 +751,26 @@ static int ensure_rev(DevPlan &pl) {
     EGG_TRY(upload_vec(&pl.d_segs, segs.data(), segs.size() * sizeof(RevSegDev), &acc));
     EGG_TRY(upload_vec(&pl.d_qoff, rp.qoff.data(), rp.qoff.size() * sizeof(int64_t), &acc));
     EGG_TRY(upload_vec(&pl.d_qseg, rp.qseg.data(), rp.qseg.size() * sizeof(int32_t), &acc));
+    std::vector<ParTaskDev> tasks(rp.tasks.size());
+    for (size_t i = 0; i < tasks.size(); i++) {
+        const ParTask &t = rp.tasks[i];
+        tasks[i] = ParTaskDev{t.start, t.row0, t.len, t.p, t.level, 0};
+    }
+    EGG_TRY(upload_vec(&pl.d_prank, rp.prank.data(), rp.prank.size() * sizeof(int32_t), &acc));
+    EGG_TRY(upload_vec(&pl.d_tasks, tasks.data(), tasks.size() * sizeof(ParTaskDev), &acc));
+    EGG_TRY(upload_vec(&pl.d_roff, rp.roff.data(), rp.roff.size() * sizeof(int64_t), &acc));
+    EGG_TRY(upload_vec(&pl.d_rtask, rp.rtask.data(), rp.rtask.size() * sizeof(int32_t), &acc));
+    EGG_TRY(upload_vec(&pl.d_qrow, rp.qrow.data(), rp.qrow.size() * sizeof(int32_t), &acc));
+    EGG_TRY(upload_vec(&pl.d_qcol, rp.qcol.data(), rp.qcol.size() * sizeof(int32_t), &acc));
+    std::vector<RedRowDev> red(rp.red.size());
+    for (size_t i = 0; i < red.size(); i++)
+        red[i] = RedRowDev{rp.red[i].tb, rp.red[i].te, rp.red[i].u, 0};
+    EGG_TRY(upload_vec(&pl.d_red, red.data(), red.size() * sizeof(RedRowDev), &acc));
+    EGG_TRY(upload_vec(&pl.d_srow0, rp.srow0.data(), rp.srow0.size() * sizeof(int32_t), &acc));
+    EGG_TRY(upload_vec(&pl.d_sruns, rp.sruns.data(), rp.sruns.size() * sizeof(int32_t), &acc));
+    pl.lvl_task = rp.lvl_task;
+    pl.part_rows = rp.part_rows;
+    pl.nred = (int64_t)rp.red.size();
     pl.lvl_seg = rp.lvl_seg;
     pl.nseg = (int64_t)rp.segs.size();
     pl.rev_bytes = acc;
@@ -710,12 +779,93 @@ static int ensure_rev(DevPlan &pl) {
 }
 
 // Device bytes one instance needs for forward + reverse: every level of H, two levels of G,
-// the segment partials, and its P and dP columns.
+// the dP partials (segment sums, or task rows plus stage-one rows), and its P and dP columns.
 static size_t grad_bytes_per_instance(const DevPlan &pl, int dtype) {
     const size_t hb = elt_hbytes(dtype), pb = elt_pbytes(dtype);
     const PlanLayout &lay = pl.lay;
-    return (size_t)(lay.nstates + 2 * lay.max_level) * hb + (size_t)std::max<int64_t>(pl.nseg, 1) * hb +
+    return (size_t)(lay.nstates + 2 * lay.max_level) * hb +
+           (size_t)std::max<int64_t>(std::max<int64_t>(pl.nseg, pl.part_rows + pl.nred), 1) * hb +
            (size_t)lay.npairs * (pb + hb);
+}
+
+// Waves per workgroup of the parent-major dP kernel at level L: the most (of 8, 4, 2) whose
+// LDS rows fit -- each wave holds rows p+1..K of dP for 64 instances, p >= p_lo -- or 0 if none
+// does.  The choice depends on the degree, the element type and the device only, so the
+// summation order (each wave takes a fixed slice of a task's parents) does too.  A wave's slice
+// is at most 64 parents (one rank per lane).
+static int par_waves(const DevCaps &caps, const PlanLayout &lay, int L, int dtype, int *rows,
+                     size_t *lds) {
+    *rows = lay.K - lay.lo[L - 1];
+    for (int w = kParWaves; w >= 2; w /= 2) {
+        *lds = (size_t)w * (*rows) * 64 * elt_hbytes(dtype);
+        if (*lds <= caps.max_lds_block && (kParTaskLen + w - 1) / w <= 64) return w;
+    }
+    return 0;
+}
+
+// The parent-major dP path needs the Int32 layout and every level's rows in LDS.
+static bool use_par_dp(const DevCaps &caps, const DevPlan &pl, int dtype) {
+    if (pl.lay.index_bytes != 4 || pl.lvl_task.empty()) return false;
+    for (int L = 2; L <= pl.lay.K / 2 + 1; L++) {
+        int rows;
+        size_t lds;
+        if (!par_waves(caps, pl.lay, L, dtype, &rows, &lds)) return false;
+    }
+    return true;
+}
+
+template <int DT, int WPB, int U>
+static void launch_dp_par_wu(const DevCaps &caps, const DevPlan &pl, int L, int rows, size_t lds,
+                             const void *Hc, const void *Gs, void *part, int64_t nb, hipStream_t st) {
+    using HT = typename Elt<DT>::H;
+    const PlanLayout &lay = pl.lay;
+    auto kern = k_rev_dp_par<DT, WPB, U>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)caps.max_lds_block);
+        (void)hipGetLastError();
+        attr_set = true;
+    }
+    const int64_t t0 = pl.lvl_task[L - 1], nt = pl.lvl_task[L] - t0;
+    const int64_t ntiles = (nb + 63) / 64;
+    if (nt == 0) return;
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, nt * ntiles));
+    kern<<<dim3((unsigned)blocks), dim3(WPB * 64), lds, st>>>(
+        (const int32_t *)pl.d_trans + lay.tstart[L - 1], lay.size[L - 1] - 1, lay.K,
+        (const int32_t *)pl.d_prank, (const ParTaskDev *)pl.d_tasks, t0, nt, (const HT *)Hc,
+        lay.levels[L - 2], nb, (const HT *)Gs, nb, (HT *)part, nb, ntiles, rows,
+        (kParTaskLen + WPB - 1) / WPB);
+}
+
+// Transitions per batch: for out-degree k <= 16 the smallest of 8 / 12 / 16 that holds a whole
+// parent; above that, batches of 12 up to k = 24 and of 16 beyond, which keeps the repeated
+// tail of the last batch short.  16-byte elements always use 8 (their registers are twice as
+// wide).
+template <int DT, int WPB>
+static void launch_dp_par_w(const DevCaps &caps, const DevPlan &pl, int L, int rows, size_t lds,
+                            const void *Hc, const void *Gs, void *part, int64_t nb, hipStream_t st) {
+    const int k = pl.lay.size[L - 1] - 1;
+    if constexpr (sizeof(typename Elt<DT>::H) > 8) {
+        launch_dp_par_wu<DT, WPB, 8>(caps, pl, L, rows, lds, Hc, Gs, part, nb, st);
+    } else {
+        if (k <= 8) launch_dp_par_wu<DT, WPB, 8>(caps, pl, L, rows, lds, Hc, Gs, part, nb, st);
+        else if (k <= 12 || (k > 16 && k <= 24))
+            launch_dp_par_wu<DT, WPB, 12>(caps, pl, L, rows, lds, Hc, Gs, part, nb, st);
+        else launch_dp_par_wu<DT, WPB, 16>(caps, pl, L, rows, lds, Hc, Gs, part, nb, st);
+    }
+}
+
+template <int DT>
+static void launch_dp_par(const DevCaps &caps, const DevPlan &pl, int L, const void *Hc,
+                          const void *Gs, void *part, int64_t nb, hipStream_t st) {
+    int rows;
+    size_t lds;
+    switch (par_waves(caps, pl.lay, L, DT, &rows, &lds)) {
+    case 8: launch_dp_par_w<DT, 8>(caps, pl, L, rows, lds, Hc, Gs, part, nb, st); break;
+    case 4: launch_dp_par_w<DT, 4>(caps, pl, L, rows, lds, Hc, Gs, part, nb, st); break;
+    case 2: launch_dp_par_w<DT, 2>(caps, pl, L, rows, lds, Hc, Gs, part, nb, st); break;
+    }
 }
 
 template <int DT, typename W>
@@ -764,9 +914,10 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
     EGG_TRY(ensure_rev(pl));
     const size_t hb = sizeof(HT), pb = sizeof(typename E::P);
     const int64_t chunk = tune.chunk > 0 ? std::min(tune.chunk, B) : B;
+    const bool par = sizeof(W) == 4 && use_par_dp(ds.caps, pl, DT);
     EGG_TRY(ensure(ds.Hg, (size_t)lay.nstates * chunk * hb));
     EGG_TRY(ensure(ds.G, (size_t)2 * lay.max_level * chunk * hb));
-    EGG_TRY(ensure(ds.part, (size_t)std::max<int64_t>(pl.nseg, 1) * chunk * hb));
+    EGG_TRY(ensure(ds.part, (size_t)std::max<int64_t>(par ? pl.part_rows + pl.nred : pl.nseg, 1) * chunk * hb));
     HT *H = (HT *)ds.Hg.p;
     HT *G[2] = {(HT *)ds.G.p, (HT *)ds.G.p + (size_t)lay.max_level * chunk};
     HT *part = (HT *)ds.part.p;
@@ -788,21 +939,36 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
         k_fill_one<DT><<<dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st>>>(G[top & 1], nb);
         const int64_t ntiles = (nb + 63) / 64;
         for (int L = top; L >= 2; L--) {
-            const int64_t s0 = pl.lvl_seg[L - 1], ns = pl.lvl_seg[L] - s0;
-            if (ns > 0) {
-                const int64_t tasks = ns * ntiles;
-                const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (tasks + 3) / 4));
-                k_rev_dp_seg<DT><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
-                    (const uint64_t *)pl.d_pent, (const RevSegDev *)pl.d_segs, s0, ns,
-                    level_rows(L - 1), nb, G[L & 1], nb, part, nb, ntiles);
+            if (par) {
+                launch_dp_par<DT>(ds.caps, pl, L, level_rows(L - 1), G[L & 1], part, nb, st);
+            } else {
+                const int64_t s0 = pl.lvl_seg[L - 1], ns = pl.lvl_seg[L] - s0;
+                if (ns > 0) {
+                    const int64_t tasks = ns * ntiles;
+                    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (tasks + 3) / 4));
+                    k_rev_dp_seg<DT><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
+                        (const uint64_t *)pl.d_pent, (const RevSegDev *)pl.d_segs, s0, ns,
+                        level_rows(L - 1), nb, G[L & 1], nb, part, nb, ntiles);
+                }
             }
             if (L > 2)
                 launch_rev_g<DT, W>(ds.caps, tune, pl, L, Pc, ldp, G[L & 1], G[(L - 1) & 1], nb, st);
         }
         const int64_t total = lay.npairs * nb;
         const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (total + 255) / 256));
-        k_rev_dp_reduce<DT><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
-            (const int64_t *)pl.d_qoff, (const int32_t *)pl.d_qseg, part, dPc, lddp, lay.npairs, nb);
+        if (par) {
+            HT *runs = part + (size_t)pl.part_rows * nb;
+            const int64_t rtotal = pl.nred * nb;
+            const int64_t rblocks = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (rtotal + 255) / 256));
+            k_rev_dp_reduce_runs<DT><<<dim3((unsigned)rblocks), dim3(256), 0, st>>>(
+                (const RedRowDev *)pl.d_red, pl.nred, (const int32_t *)pl.d_rtask,
+                (const ParTaskDev *)pl.d_tasks, part, runs, nb);
+            k_rev_dp_reduce_pairs<DT><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
+                (const int32_t *)pl.d_qrow, (const int32_t *)pl.d_qcol, (const int32_t *)pl.d_srow0,
+                (const int32_t *)pl.d_sruns, lay.K, runs, dPc, lddp, lay.npairs, nb);
+        } else
+            k_rev_dp_reduce<DT><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
+                (const int64_t *)pl.d_qoff, (const int32_t *)pl.d_qseg, part, dPc, lddp, lay.npairs, nb);
     }
     EGG_HIP(hipGetLastError());
     return EGGMAN_OK;
@@ -827,6 +993,15 @@ static int forward_grad_any(int dtype, DevState &ds, DevPlan &pl, const void *P,
     case EGGMAN_I128: return forward_grad_dt<EGGMAN_I128>(ds, pl, P, ldp, B, out, dP, lddp, st, tune);
     }
     return fail(EGGMAN_EINVAL, "unknown dtype %d", dtype);
+}
+
+// Instances per pass of the gradient: the forward's tuned pass raised to at least kGradMinPass
+// (the reverse kernels spread a level over (task, 64-instance tile) pairs and want eight or more
+// tiles to fill the device), within the batch and the memory cap.
+constexpr int64_t kGradMinPass = 512;
+static int64_t grad_pass(const Tune &t, int64_t B, int64_t cap) {
+    const int64_t c = std::max<int64_t>(t.chunk > 0 ? t.chunk : B, kGradMinPass);
+    return std::max<int64_t>(1, std::min(std::min(c, B), cap));
 }
 
 // Largest chunk of instances whose gradient buffers fit in `frac` of the free device memory.
@@ -1314,8 +1489,7 @@ int eggman_dp_forward_mixed(int ngroups, const uint64_t *plans, const int64_t *c
     EGG_TRY(ensure(ds->P, (size_t)np_total * pb));
     EGG_TRY(ensure(ds->out, (size_t)total * hb));
     EGG_HIP(hipMemcpyAsync(ds->P.p, P_host, (size_t)np_total * pb, hipMemcpyHostToDevice, ds->stream));
-    Tune tune;
-    EGG_TRY(forward_mixed_any(dtype, *ds, pls, cnt, ds->P.p, ds->out.p, ds->stream, tune));
+    EGG_TRY(forward_mixed_any(dtype, *ds, pls, cnt, ds->P.p, ds->out.p, ds->stream));
     EGG_HIP(hipMemcpyAsync(out_host, ds->out.p, (size_t)total * hb, hipMemcpyDeviceToHost, ds->stream));
     EGG_HIP(hipStreamSynchronize(ds->stream));
     return EGGMAN_OK;
@@ -1337,8 +1511,7 @@ int eggman_dp_forward_mixed_device(int ngroups, const uint64_t *plans, const int
     EGG_TRY(dev_state(pls[0]->device, &ds));
     hipStream_t st = stream ? (hipStream_t)stream : ds->stream;
     EGG_TRY(chain_stream(*ds, st));
-    Tune tune;
-    return forward_mixed_any(dtype, *ds, pls, cnt, P_dev, out_dev, st, tune);
+    return forward_mixed_any(dtype, *ds, pls, cnt, P_dev, out_dev, st);
 }
 
 int eggman_dp_forward_grad(uint64_t plan, int dtype, const void *P_host, int64_t B,
@@ -1365,8 +1538,9 @@ int eggman_dp_forward_grad(uint64_t plan, int dtype, const void *P_host, int64_t
     EGG_TRY(ensure(ds->P, (size_t)chunk * npairs * pb));
     EGG_TRY(ensure(ds->dP, (size_t)chunk * npairs * hb));
     EGG_TRY(ensure(ds->out, (size_t)chunk * hb));
-    // the forward's tuned choices (instances per pass, waves per workgroup) serve the gradient
-    const Tune tune = pick_tune(*pl, dtype, chunk);
+    // the forward's tuned choices (waves per workgroup, LDS staging) serve the gradient
+    Tune tune = pick_tune(*pl, dtype, chunk);
+    tune.chunk = grad_pass(tune, chunk, chunk);
     std::vector<char> stage, gstage;
     for (int64_t b0 = 0; b0 < B; b0 += chunk) {
         const int64_t nb = std::min(chunk, B - b0);
@@ -1422,7 +1596,8 @@ int eggman_dp_forward_grad_device(uint64_t plan, int dtype, const void *P_dev, i
     EGG_TRY(ensure_rev(*pl));
     int64_t chunk = B;
     EGG_TRY(grad_chunk(*pl, dtype, B, &chunk));
-    const Tune tune = pick_tune(*pl, dtype, chunk);
+    Tune tune = pick_tune(*pl, dtype, chunk);
+    tune.chunk = grad_pass(tune, B, chunk);
     return forward_grad_any(dtype, *ds, *pl, P_dev, ldp, B, out_dev, dP_dev, lddp, st, tune);
 }
 
@@ -1454,7 +1629,9 @@ int eggman_cache_clear(void) {
     for (auto &kv : g_plans) {
         DeviceGuard g(kv.second->device);
         DevPlan &p = *kv.second;
-        for (void *ptr : {p.d_trans, p.d_cent, p.d_coff, p.d_pent, p.d_segs, p.d_qoff, p.d_qseg})
+        for (void *ptr : {p.d_trans, p.d_cent, p.d_coff, p.d_pent, p.d_segs, p.d_qoff, p.d_qseg,
+                          p.d_prank, p.d_tasks, p.d_roff, p.d_rtask, p.d_qrow, p.d_qcol,
+                          p.d_red, p.d_srow0, p.d_sruns})
             if (ptr) (void)hipFree(ptr);
     }
     g_plans.clear();

@@ -152,6 +152,200 @@ __global__ __launch_bounds__(256) void k_rev_dp_seg(
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Parent-major dP for one level.  Every transition of a parent S leaves from p = min(S), so all
+// of S's contributions H[child] * G[S] land in row p of dP, at column j - p - 1 for the partner
+// j.  One workgroup takes a task (parents of the level with the same p) for one 64-instance
+// tile; wave w accumulates the row over its fixed slice of the task's parents in its own LDS
+// rows -- one G read per parent and one child read per transition, half the gathers of the
+// pair-major segments -- and the waves' rows are then summed in wave order into the task's
+// partial rows.  A parent's transitions have distinct partners, so a batch of them reads all its
+// LDS slots before writing any: the read-modify-writes of a batch overlap instead of chaining.
+// Tasks of a row are summed in a fixed order by k_rev_dp_reduce_runs / k_rev_dp_reduce_pairs.
+struct ParTaskDev {
+    int64_t start;
+    int64_t row0;
+    int32_t len;
+    int32_t p;
+    int32_t level;
+    int32_t pad;
+};
+
+struct RedRowDev {
+    int32_t tb, te, u, pad;
+};
+
+// Plan words are int32 here (the parent-major path runs on the Int32 layout).  U is the batch of
+// transitions gathered together: parents of out-degree k <= U go two at a time (2U gathers in
+// flight); larger out-degrees take ceil(k / U) batches per parent.  Plan words and parent ranks are read with
+// vector loads, one word per lane, and broadcast with readlane, so the only LDS-counter traffic
+// in the loop is the accumulator's own.
+template <int DT, int WPB, int U>
+__global__ __launch_bounds__(WPB * 64) void k_rev_dp_par(
+    const int32_t *__restrict__ trans_lvl, int k, int K,
+    const int32_t *__restrict__ prank, const ParTaskDev *__restrict__ tasks, int64_t task0,
+    int64_t ntask, const typename Elt<DT>::H *__restrict__ Hc, int64_t cfirst, int64_t ldh,
+    const typename Elt<DT>::H *__restrict__ Gs, int64_t ldg,
+    typename Elt<DT>::H *__restrict__ part, int64_t nb, int64_t ntiles, int rows_per_wave,
+    int sub_len) {
+    using E = Elt<DT>;
+    using HT = typename E::H;
+    constexpr int bits = Packing<int32_t>::bits;
+    constexpr int32_t mask = (1 << bits) - 1;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    HT *rows = reinterpret_cast<HT *>(lds_raw);
+    const int lane = (int)(threadIdx.x & 63);
+    // uniform across the wave
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int64_t wslice = (int64_t)rows_per_wave * 64;
+    HT *acc = rows + wave * wslice;
+    const int64_t nwork = ntask * ntiles;
+    const int klast = k - 1;
+    for (int64_t w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const int64_t ti = w / ntiles, tile = w - ti * ntiles;
+        const ParTaskDev tk = tasks[task0 + ti];
+        const int64_t b = tile * 64 + lane;
+        // lanes past the batch shadow the last instance (valid addresses); their sums are dropped
+        const int64_t bl = b < nb ? b : nb - 1;
+        const int nrow = K - tk.p;
+        // 0-based index of pair (p, p+1): (p-1)K - p(p+1)/2 + p
+        const int qbase = (tk.p - 1) * K - tk.p * (tk.p + 1) / 2 + tk.p;
+        for (int u = 0; u < nrow; u++) acc[u * 64 + lane] = E::zero();
+        // this wave's parents: [wave * sub_len, (wave + 1) * sub_len) of the task
+        const int i0 = wave * sub_len;
+        const int cnt = min(tk.len - i0, sub_len);
+        if (cnt > 0) {
+            const int rk = prank[tk.start + i0 + min(lane, cnt - 1)];   // lane j: rank of parent j
+            auto words = [&](int j) {   // lane v: word v of parent j (clamped to the last)
+                const int r = __builtin_amdgcn_readlane(rk, j);
+                return trans_lvl[(int64_t)r * k + min(lane, klast)];
+            };
+            auto gval = [&](int j) {
+                return Gs[(int64_t)__builtin_amdgcn_readlane(rk, j) * ldg + bl];
+            };
+            auto gather = [&](int wv, int t, HT *h) {
+#pragma unroll
+                for (int v = 0; v < U; v++) {
+                    const int wd = __builtin_amdgcn_readlane(wv, min(t + v, klast));
+                    h[v] = Hc[((int64_t)(wd >> bits) - cfirst) * ldh + bl];
+                }
+            };
+            // `on` false stores back what was read (a no-op written without a branch, so the
+            // loads feeding it cannot be sunk below earlier updates)
+            auto update = [&](int wv, int t, const HT *h, HT g, bool on) {
+                // a short batch repeats its last transition: the repeats read the same slot
+                // before any write and store the same value, so they change nothing
+                HT a[U];
+                int slot[U];
+#pragma unroll
+                for (int v = 0; v < U; v++) {
+                    const int wd = __builtin_amdgcn_readlane(wv, min(t + v, klast));
+                    slot[v] = ((wd & mask) - 1 - qbase) * 64 + lane;
+                    a[v] = acc[slot[v]];
+                }
+#pragma unroll
+                for (int v = 0; v < U; v++) {
+                    const HT r = E::muladd_hh(a[v], h[v], g);
+                    acc[slot[v]] = on ? r : a[v];
+                }
+            };
+            if (k <= U) {
+                // two parents per step: both parents' gathers are issued together, then the
+                // first parent's updates, then the second's (so slots shared by the two parents
+                // are summed in parent order, as one parent at a time would)
+                for (int j = 0; j < cnt; j += 2) {
+                    const bool two = j + 1 < cnt;
+                    const int wa = words(j), wb = words(two ? j + 1 : j);
+                    const HT ga = gval(j), gb = gval(two ? j + 1 : j);
+                    HT ha[U], hb[U];
+                    gather(wa, 0, ha);
+                    gather(wb, 0, hb);
+                    // every gather of the step is issued before the first LDS update
+                    __builtin_amdgcn_sched_barrier(0);
+                    update(wa, 0, ha, ga, true);
+                    update(wb, 0, hb, gb, two);
+                }
+            } else {
+                // out-degree above 64 does not occur (degree <= 64)
+                for (int j = 0; j < cnt; j++) {
+                    const int wv = words(j);
+                    const HT g = gval(j);
+                    for (int t = 0; t < k; t += U) {
+                        HT h[U];
+                        gather(wv, t, h);
+                        update(wv, t, h, g, true);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // the task's row: the waves' rows summed in wave order, spread over the workgroup
+        for (int e = (int)threadIdx.x; e < nrow * 64; e += WPB * 64) {
+            const int64_t bb = tile * 64 + (e & 63);
+            if (bb < nb) {
+                HT sum = rows[e];
+#pragma unroll
+                for (int v = 1; v < WPB; v++) sum = E::add_h(sum, rows[v * wslice + e]);
+                part[(tk.row0 + (e >> 6)) * nb + bb] = sum;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Sum of `n` values src[idx(i)] in ascending i, loads issued eight at a time.
+template <int DT, typename F>
+__device__ __forceinline__ typename Elt<DT>::H ordered_sum(int64_t n, F &&load) {
+    using E = Elt<DT>;
+    typename E::H acc = E::zero();
+    int64_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+        typename E::H v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = load(i + j);
+#pragma unroll
+        for (int j = 0; j < 8; j++) acc = E::add_h(acc, v[j]);
+    }
+    for (; i < n; i++) acc = E::add_h(acc, load(i));
+    return acc;
+}
+
+// Stage one: red[r][b] = sum of column u of the partial rows of tasks rtask[tb .. te), in order.
+template <int DT>
+__global__ __launch_bounds__(256) void k_rev_dp_reduce_runs(
+    const RedRowDev *__restrict__ red, int64_t nred, const int32_t *__restrict__ rtask,
+    const ParTaskDev *__restrict__ tasks, const typename Elt<DT>::H *__restrict__ part,
+    typename Elt<DT>::H *__restrict__ out, int64_t nb) {
+    const int64_t total = nred * nb;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const int64_t r = i / nb, b = i - r * nb;
+        const RedRowDev d = red[r];
+        out[r * nb + b] = ordered_sum<DT>(d.te - d.tb, [&](int64_t t) {
+            return part[(tasks[rtask[d.tb + t]].row0 + d.u) * nb + b];
+        });
+    }
+}
+
+// Stage two: dP[q][b] = sum over the runs of row p(q), in order, of their stage-one rows.
+template <int DT>
+__global__ __launch_bounds__(256) void k_rev_dp_reduce_pairs(
+    const int32_t *__restrict__ qrow, const int32_t *__restrict__ qcol,
+    const int32_t *__restrict__ srow0, const int32_t *__restrict__ sruns, int K,
+    const typename Elt<DT>::H *__restrict__ red_out, typename Elt<DT>::H *__restrict__ dP,
+    int64_t lddp, int64_t npairs, int64_t nb) {
+    const int64_t total = npairs * nb;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const int64_t q = i / nb, b = i - q * nb;
+        const int p = qrow[q], u = qcol[q];
+        const int64_t r0 = srow0[p] + u, step = K - p;
+        dP[q * lddp + b] = ordered_sum<DT>(sruns[p], [&](int64_t g) {
+            return red_out[(r0 + g * step) * nb + b];
+        });
+    }
+}
+
 // dP[q][b] = sum of the pair's segment sums in ascending level order.
 template <int DT>
 __global__ __launch_bounds__(256) void k_rev_dp_reduce(

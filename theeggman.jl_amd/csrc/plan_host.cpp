@@ -134,7 +134,8 @@ int build_plan_host(const PlanLayout &lay, void *starts, void *trans, void *leve
 }
 
 template <typename W>
-static int build_rev_t(const PlanLayout &lay, const W *trans, int seg_len, RevPlanHost *rp) {
+static int build_rev_t(const PlanLayout &lay, const W *trans, int seg_len, int task_len,
+                       int fan_in, RevPlanHost *rp) {
     const int64_t mask = ((int64_t)1 << lay.bits) - 1;
     rp->coff.assign(lay.nstates + 1, 0);
     rp->cent.assign((size_t)lay.ntrans * sizeof(W), 0);
@@ -190,13 +191,93 @@ static int build_rev_t(const PlanLayout &lay, const W *trans, int seg_len, RevPl
         rp->qseg.insert(rp->qseg.end(), per_pair[q].begin(), per_pair[q].end());
     }
     rp->qoff[lay.npairs] = (int64_t)rp->qseg.size();
+
+    // parent-major dP tasks
+    const int K = lay.K;
+    rp->qrow.assign(lay.npairs, 0);
+    rp->qcol.assign(lay.npairs, 0);
+    for (int i = 1; i < K; i++)
+        for (int j = i + 1; j <= K; j++) {
+            const int64_t q = pair_index(i, j, K) - 1;
+            rp->qrow[q] = i;
+            rp->qcol[q] = j - i - 1;
+        }
+    rp->prank.assign(lay.nstates, 0);
+    rp->tasks.clear();
+    rp->lvl_task.assign(lay.nlevels, 0);
+    std::vector<std::vector<int32_t>> row_tasks(K + 1);
+    int64_t rows = 0;
+    for (int L = 2; L <= K / 2 + 1; L++) {
+        rp->lvl_task[L - 1] = (int64_t)rp->tasks.size();
+        const int k = lay.size[L - 1] - 1;
+        const int64_t n = lay.nst[L - 1], base = lay.levels[L - 1] - 1, t0 = lay.tstart[L - 1];
+        std::vector<std::vector<int32_t>> byp(K + 1);
+        for (int64_t r = 0; r < n; r++) {
+            const int64_t q = ((int64_t)trans[t0 + r * k] & mask) - 1;
+            // the LDS row of a task holds row p only, and a batch of a parent's transitions
+            // updates its slots together: the pairs must share the row and strictly ascend
+            for (int v = 1; v < k; v++) {
+                const int64_t qv = ((int64_t)trans[t0 + r * k + v] & mask) - 1;
+                const int64_t qu = ((int64_t)trans[t0 + r * k + v - 1] & mask) - 1;
+                if (rp->qrow[qv] != rp->qrow[q] || qv <= qu) {
+                    rp->tasks.clear();
+                    rp->lvl_task.clear();
+                    return EGGMAN_OK;
+                }
+            }
+            byp[rp->qrow[q]].push_back((int32_t)r);
+        }
+        int64_t pos = base;
+        for (int p = 1; p <= K; p++) {
+            const auto &v = byp[p];
+            for (size_t a = 0; a < v.size(); a += task_len) {
+                ParTask tk;
+                tk.start = pos + (int64_t)a;
+                tk.len = (int32_t)std::min<size_t>(task_len, v.size() - a);
+                tk.p = p;
+                tk.level = L;
+                tk.row0 = rows;
+                tk.pad = 0;
+                rows += K - p;
+                row_tasks[p].push_back((int32_t)rp->tasks.size());
+                rp->tasks.push_back(tk);
+            }
+            for (size_t a = 0; a < v.size(); a++) rp->prank[pos + (int64_t)a] = v[a];
+            pos += (int64_t)v.size();
+        }
+    }
+    rp->lvl_task[lay.nlevels - 1] = (int64_t)rp->tasks.size();
+    rp->part_rows = rows;
+    rp->roff.assign(K + 2, 0);
+    rp->rtask.clear();
+    for (int p = 1; p <= K; p++) {
+        rp->roff[p] = (int64_t)rp->rtask.size();
+        rp->rtask.insert(rp->rtask.end(), row_tasks[p].begin(), row_tasks[p].end());
+    }
+    rp->roff[K + 1] = (int64_t)rp->rtask.size();
+    // two-stage reduction of the partial rows
+    rp->red.clear();
+    rp->srow0.assign(K + 2, 0);
+    rp->sruns.assign(K + 2, 0);
+    for (int p = 1; p <= K; p++) {
+        rp->srow0[p] = (int32_t)rp->red.size();
+        const int64_t tb = rp->roff[p], te = rp->roff[p + 1];
+        for (int64_t a = tb; a < te; a += fan_in) {
+            rp->sruns[p]++;
+            for (int u = 0; u < K - p; u++)
+                rp->red.push_back(RedRow{(int32_t)a, (int32_t)std::min<int64_t>(te, a + fan_in), u, 0});
+        }
+    }
     return EGGMAN_OK;
 }
 
-int build_rev_plan(const PlanLayout &lay, const void *trans, int seg_len, RevPlanHost *rp) {
-    if (seg_len < 1) return fail(EGGMAN_EINVAL, "segment length must be positive");
-    if (lay.index_bytes == 4) return build_rev_t<int32_t>(lay, (const int32_t *)trans, seg_len, rp);
-    return build_rev_t<int64_t>(lay, (const int64_t *)trans, seg_len, rp);
+int build_rev_plan(const PlanLayout &lay, const void *trans, int seg_len, int task_len,
+                   int fan_in, RevPlanHost *rp) {
+    if (seg_len < 1 || task_len < 1 || fan_in < 1)
+        return fail(EGGMAN_EINVAL, "segment, task and fan-in lengths must be positive");
+    if (lay.index_bytes == 4)
+        return build_rev_t<int32_t>(lay, (const int32_t *)trans, seg_len, task_len, fan_in, rp);
+    return build_rev_t<int64_t>(lay, (const int64_t *)trans, seg_len, task_len, fan_in, rp);
 }
 
 // Check that caller-supplied arrays are a plan the kernels can walk safely: the closed-form

@@ -35,6 +35,28 @@ struct RevSeg {
     int32_t level;   // 1-based level of the parents
 };
 
+// Parent-major dP tasks: the parents of one level with the same p = min(S), at most `len` of
+// them, whose contributions H[child] * G[parent] all land in row p of dP (pairs (p, j), j > p).
+// A task accumulates that row for its parents and writes it to rows [row0, row0 + K - p) of
+// the partial buffer.
+struct ParTask {
+    int64_t start;   // first entry in prank
+    int64_t row0;    // first partial row of this task
+    int32_t len;     // parents
+    int32_t p;       // 1-based row of dP
+    int32_t level;   // 1-based level of the parents
+    int32_t pad;
+};
+
+// The partial rows of dP row p are summed in two fixed-order stages: stage one sums runs of at
+// most `fan_in` consecutive tasks of the row (ascending level, then ascending parent), one
+// stage-one row per (run, column u); stage two sums a pair's stage-one rows in run order.
+struct RedRow {
+    int32_t tb, te;  // tasks rtask[tb .. te)
+    int32_t u;       // column j - p - 1
+    int32_t pad;
+};
+
 struct RevPlanHost {
     std::vector<int64_t> coff;     // nstates + 1
     std::vector<uint8_t> cent;     // ntrans words of index_bytes each
@@ -43,9 +65,22 @@ struct RevPlanHost {
     std::vector<int64_t> lvl_seg;  // first segment of each level (1-based L at [L-1]), + total
     std::vector<int64_t> qoff;     // npairs + 1
     std::vector<int32_t> qseg;     // segments of each pair, ascending level
+    // parent-major dP (kernels_rev.hpp: k_rev_dp_par)
+    std::vector<int32_t> prank;    // per level, its parents' ranks grouped by p (state-indexed)
+    std::vector<ParTask> tasks;
+    std::vector<int64_t> lvl_task; // first task of each level, + total
+    std::vector<int64_t> roff;     // tasks of each row p (index p, 1..K), ascending level
+    std::vector<int32_t> rtask;
+    std::vector<int32_t> qrow;     // 1-based row p of each 0-based pair index
+    std::vector<int32_t> qcol;     // offset j - p - 1 of each pair inside its row
+    int64_t part_rows = 0;
+    std::vector<RedRow> red;       // stage-one rows
+    std::vector<int32_t> srow0;    // first stage-one row of each row p (index p, 1..K)
+    std::vector<int32_t> sruns;    // runs of each row p
 };
 
-int build_rev_plan(const PlanLayout &lay, const void *trans, int seg_len, RevPlanHost *rp);
+int build_rev_plan(const PlanLayout &lay, const void *trans, int seg_len, int task_len,
+                   int fan_in, RevPlanHost *rp);
 
 const Binom &host_binom();
 int pair_bits_for(int index_bytes);

@@ -26,6 +26,7 @@ def runtime():
         H.hipMalloc.argtypes = [ctypes.POINTER(vp), sz]
         H.hipFree.argtypes = [vp]
         H.hipMemcpy.argtypes = [vp, vp, sz, ctypes.c_int]
+        H.hipMemcpyAsync.argtypes = [vp, vp, sz, ctypes.c_int, vp]
         H.hipMemset.argtypes = [vp, ctypes.c_int, sz]
         H.hipStreamCreate.argtypes = [ctypes.POINTER(vp)]
         H.hipStreamDestroy.argtypes = [vp]
@@ -128,6 +129,12 @@ def copy_d2d(dst: DeviceBuffer, src: DeviceBuffer, nbytes: int) -> None:
     if nbytes > min(dst.nbytes, src.nbytes):
         raise ValueError("copy larger than a buffer")
     _chk(runtime().hipMemcpy(dst.ptr, src.ptr, nbytes, D2D), "hipMemcpy D2D")
+
+
+def copy_d2d_async(dst: DeviceBuffer, src: DeviceBuffer, nbytes: int, stream: "Stream") -> None:
+    if nbytes > min(dst.nbytes, src.nbytes):
+        raise ValueError("copy larger than a buffer")
+    _chk(runtime().hipMemcpyAsync(dst.ptr, src.ptr, nbytes, D2D, stream.handle), "hipMemcpyAsync D2D")
 
 
 def device_synchronize() -> None:
