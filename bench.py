@@ -353,10 +353,26 @@ def main():
             result[name] = {"ms_per_hafnian": ms1, "wall_ms_per_hafnian": wall1,
                             "hafnians_per_s": 1e3 / ms1, "transitions_per_s": nt / (ms1 / 1e3),
                             "tuned_launch": choice}
+            if name == "plain_f64" and not args.no_grad:
+                # one matrix per call, value and full gradient
+                dP1 = hip.DeviceBuffer(8 * (K * (K - 1) // 2))
+                h1 = te._device_plan(be, plan)
+
+                def one_grad():
+                    _lib.check(_lib.lib().eggman_dp_forward_grad_device(
+                        h1, _lib.F64, P1d.addr, 1, 1, o1.addr, dP1.addr, 1, stream.handle), "grad")
+                log("plain_f64 with gradient")
+                msg1, _ = time_device(hip, stream, one_grad, max(args.steps, 20), args.warmup)
+                result["plain_grad_f64"] = {"ms_per_hafnian_with_gradient": msg1,
+                                            "hafnians_with_gradient_per_s": 1e3 / msg1}
         result["batched_vs_plain_f64_hafnians"] = result["hafnians_per_s"] / result["plain_f64"]["hafnians_per_s"]
         if "fwd_grad" in result:
             result["batched_grad_vs_plain_f64_hafnians"] = (
                 result["fwd_grad"]["hafnians_with_gradient_per_s"] / result["plain_f64"]["hafnians_per_s"])
+        if "fwd_grad" in result and "plain_grad_f64" in result:
+            result["batched_grad_vs_plain_grad_f64_hafnians"] = (
+                result["fwd_grad"]["hafnians_with_gradient_per_s"]
+                / result["plain_grad_f64"]["hafnians_with_gradient_per_s"])
 
     if not args.no_cpu:
         log("cpu baseline")

@@ -184,3 +184,20 @@ def test_float_gradient_n28_sampled_deletion():
             assert abs(Gs[b, i, j] - d) <= 1e-12 * scale, (b, i, j)
         lhs = np.sum(np.triu(As[b], 1) * np.triu(Gs[b], 1))
         assert abs(lhs - N / 2 * vals[b]) <= 1e-11 * max(1.0, np.abs(As[b]).sum() * scale)
+
+
+@pytest.mark.parametrize("lo,hi", [(np.float32, np.float64), (np.complex64, np.complex128)])
+def test_single_precision_gradient_follows_double(lo, hi):
+    # the 4- and 8-byte element paths of the reverse sweep: the single-precision gradient of a
+    # matrix exactly representable in single precision agrees with the double-precision one to
+    # single-precision rounding
+    note(f"grad {np.dtype(lo)} vs {np.dtype(hi)} N=20")
+    N = 20
+    A = synth.sym_complex_normal(N, 63, 0) if hi == np.complex128 else synth.sym_uniform(N, 63, 0)
+    A = A.astype(lo).astype(hi)
+    v_lo, G_lo = te.hafnian_grad(A.astype(lo))
+    v_hi, G_hi = te.hafnian_grad(A)
+    assert G_lo.dtype == lo
+    scale = np.abs(G_hi).max()
+    assert np.abs(G_lo.astype(hi) - G_hi).max() <= 1e-4 * scale
+    assert abs(v_lo - v_hi) <= 1e-4 * max(abs(v_hi), np.abs(A).sum() * scale / N)
