@@ -199,6 +199,9 @@ int eggman_autotune(uint64_t plan, int dtype, int64_t B, int reps, char *report,
 #define EGGMAN_TUNE_FUSED 4 /* one workgroup per instance walks every level with all state
                                values in LDS (degrees whose state array fits) */
 #define EGGMAN_TUNE_WIDE 8  /* 16 waves per workgroup in the LDS kernel instead of 8 */
+#define EGGMAN_TUNE_CKPT 16 /* gradient: keep checkpoint levels only and recompute the others
+                               (chosen by itself when keeping every level would not fit a
+                               full pass in device memory) */
 
 /*
  * Set / read a tuning entry directly (e.g. to persist tuned choices across processes).  bt:
@@ -223,6 +226,19 @@ int eggman_cache_clear(void);
 /* Device memory one batch needs besides the shared plan, B*(nstates+npairs)*sizeof(T):
  * dp_batch_bytes (src/dp.jl:584-587). */
 int64_t eggman_batch_bytes(int K, int dtype, int64_t B);
+
+/*
+ * Which forward levels the gradient keeps resident for degree K (host-only query).  Level L
+ * (1 .. K/2) holds the values the reverse step at L+1 reads.  checkpointed = 0: every level
+ * (kept_rows = nstates - 1, seg_rows = 0).  checkpointed = 1: level 1 and checkpoint levels,
+ * chosen to minimise kept_rows + seg_rows, where the other levels are recomputed segment by
+ * segment in a region of seg_rows state rows.  kept[L] = 1 for a kept level (kept[0] unused,
+ * nkept >= K/2 + 1).  Rows count state values per instance.  New: the reference has no
+ * derivative; the north-star reverse sweep recomputes levels from checkpoints above a memory
+ * threshold (EGGMAN_TUNE_CKPT forces it).
+ */
+int eggman_grad_levels(int K, int checkpointed, int64_t *kept_rows, int64_t *seg_rows,
+                       uint8_t *kept, int64_t nkept);
 
 #ifdef __cplusplus
 }

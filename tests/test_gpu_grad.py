@@ -201,3 +201,58 @@ def test_single_precision_gradient_follows_double(lo, hi):
     scale = np.abs(G_hi).max()
     assert np.abs(G_lo.astype(hi) - G_hi).max() <= 1e-4 * scale
     assert abs(v_lo - v_hi) <= 1e-4 * max(abs(v_hi), np.abs(A).sum() * scale / N)
+
+
+@pytest.mark.parametrize("N", [16, 22])
+def test_exact_gradient_same_on_both_layouts(N):
+    # the Int32 layout takes the parent-major dP pass, the Int64 layout the pair-major segments;
+    # on the 128-bit integer path both must give the same integers
+    note(f"exact grad Int32 vs Int64 N={N}")
+    As = np.stack([synth.gnp_adjacency(N, 64, b) for b in range(5)]).astype(np.int64)
+    As[:, 0, 1] = As[:, 1, 0] = 3                      # a non-0/1 entry
+    P = te._gather_pairs_batch(np.int64, As, None, N)
+    v32, g32 = te._haf_dp_backend_grad(BE, P, te._dp_plan(N, np.int32))
+    v64, g64 = te._haf_dp_backend_grad(BE, P, te._dp_plan(N, np.int64))
+    assert list(v32) == list(v64)
+    assert [list(r) for r in g32] == [list(r) for r in g64]
+    for b in (0, 4):
+        assert g64[b][te._pair_index(2, N, N) - 1] == deletion(As[b], 1, N - 1, o.I128)
+
+
+@pytest.mark.parametrize("N,T,I", [(20, np.float64, np.int32), (28, np.float64, np.int32),
+                                   (18, np.complex128, np.int32), (16, np.int64, np.int32),
+                                   (20, np.float64, np.int64)])
+def test_checkpointed_gradient_is_bit_identical(N, T, I):
+    # keeping only checkpoint levels and recomputing the others must not change a bit
+    note(f"checkpointed grad N={N} {np.dtype(T)} {np.dtype(I)}")
+    B = 70
+    if T == np.int64:
+        As = np.stack([synth.gnp_adjacency(N, 65, b) for b in range(B)]).astype(np.int64)
+    elif T == np.complex128:
+        As = np.stack([synth.sym_complex_normal(N, 65, b) for b in range(B)])
+    else:
+        As = np.stack([synth.sym_uniform(N, 65, b) for b in range(B)])
+    P = te._gather_pairs_batch(T, As, None, N)
+    plan = te._dp_plan(N, I)
+    kept_rows, seg_rows, kept = te.grad_levels(N)
+    assert not all(kept[1:])                           # some level is recomputed
+    code = te.tuning.dtype_code(T)
+    ib = np.dtype(I).itemsize
+    v0, g0 = te._haf_dp_backend_grad(BE, P, plan)
+    v1, g1 = te._haf_dp_backend_grad(BE, P, plan, max_batch=16)
+
+    def entries(flags):   # the tuning entries the two call shapes look up (B=70 and B=16)
+        for bb in (B, 16):
+            _lib.check(_lib.lib().eggman_tune_set(N, ib, code, bb, 64, 0, flags, 0), "tune_set")
+    entries(te.tuning.LDS | te.tuning.CKPT)
+    try:
+        v2, g2 = te._haf_dp_backend_grad(BE, P, plan)
+        v3, g3 = te._haf_dp_backend_grad(BE, P, plan, max_batch=16)
+    finally:
+        entries(te.tuning.LDS)     # launch choices only change speed, never results
+    if T == np.int64:
+        assert list(v0) == list(v1) == list(v2) == list(v3)
+        assert [list(r) for r in g0] == [list(r) for r in g2] == [list(r) for r in g3]
+    else:
+        assert v0.tobytes() == v1.tobytes() == v2.tobytes() == v3.tobytes()
+        assert np.asarray(g0).tobytes() == np.asarray(g2).tobytes() == np.asarray(g3).tobytes()

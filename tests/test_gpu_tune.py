@@ -73,4 +73,5 @@ def test_tuning_table_round_trip(tmp_path):
     te.save_tuning(str(path))
     assert te.load_tuning(str(path)) >= 1
     assert te.tune_get(22, np.float64, 512) == {"bt": 64, "chunk": 256, "flags": 1 | 8, "occ": 2,
-                                                "lds": 1, "graph": 0, "fused": 0, "wide": 1}
+                                                "lds": 1, "graph": 0, "fused": 0, "wide": 1,
+                                                "ckpt": 0}

@@ -180,3 +180,51 @@ def test_synthetic_generator_is_stable():
     g = synth.normal(synth.stream(2, 0), 20000)
     assert abs(g.mean()) < 0.03 and abs(g.std() - 1) < 0.03
     assert (np.round(g * 2**30) == g * 2**30).all()
+
+
+def _level_rows(K):
+    """State rows of levels 1..K/2 from the closed form: level L holds the 2(L-1)-subsets of
+    {lo..K}, lo = (K - m + 1) // 2 + 1 (the counts the oracle's plan builder reproduces)."""
+    from math import comb
+    rows = [0]
+    for L in range(1, K // 2 + 1):
+        m = 2 * (L - 1)
+        lo = (K - m + 1) // 2 + 1
+        rows.append(comb(K - lo + 1, m))
+    return rows
+
+
+@pytest.mark.parametrize("K", [4, 8, 12, 16, 20, 24, 28])
+def test_gradient_checkpoints_are_optimal(K):
+    # brute force over every set of kept levels: level 1 kept, cost = kept rows + the largest
+    # run of levels that are not kept; the library's choice must reach the minimum
+    rows = _level_rows(K)
+    n = K // 2
+    assert sum(rows) == o.dp_counts(K)[0] - 1          # every level but the top one
+    best = None
+    for bits in range(1 << max(n - 1, 0)):
+        kept = [False, True] + [bool(bits >> (L - 2) & 1) for L in range(2, n + 1)]
+        seg = cur = 0
+        for L in range(2, n + 1):
+            cur = 0 if kept[L] else cur + rows[L]
+            seg = max(seg, cur)
+        cost = sum(rows[L] for L in range(1, n + 1) if kept[L]) + seg
+        best = cost if best is None else min(best, cost)
+    kr, sr, kept = te.grad_levels(K, checkpointed=True)
+    assert kept[1]
+    assert kr == sum(rows[L] for L in range(1, n + 1) if kept[L])
+    seg = cur = 0
+    for L in range(2, n + 1):
+        cur = 0 if kept[L] else cur + rows[L]
+        seg = max(seg, cur)
+    assert sr == seg
+    assert kr + sr == best
+    kf, sf, kept_f = te.grad_levels(K, checkpointed=False)
+    assert (kf, sf) == (sum(rows), 0) and all(kept_f[1:])
+
+
+def test_gradient_checkpoints_save_memory_at_large_degree():
+    for K in (28, 32, 36, 40):
+        kr, sr, _ = te.grad_levels(K, checkpointed=True)
+        full, _, _ = te.grad_levels(K, checkpointed=False)
+        assert kr + sr < 0.75 * full, K

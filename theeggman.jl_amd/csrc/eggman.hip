@@ -58,6 +58,7 @@ struct DevPlan {
     std::vector<int64_t> lvl_task;
     int64_t part_rows = 0, nred = 0;
     size_t rev_bytes = 0;
+    GradLevels gl_full, gl_ckpt;   // where the gradient keeps forward levels (plan_host.hpp)
 };
 
 // Grow-only device buffer (ext/TheEggmanKernelAbstractionsExt.jl:68-88).
@@ -211,13 +212,14 @@ struct Tune {
     bool graph = false; // replay the level sequence from a captured HIP graph
     bool fused = false; // one workgroup per instance walks every level with H in LDS (small K)
     bool wide = false;  // 16 waves per workgroup in the LDS kernel instead of 8
+    bool ckpt = false;  // gradient: keep checkpoint levels only and recompute the others
 };
 
 // Tuning flags as they cross the C ABI (eggman_tune_set / eggman_tune_get).
-enum { TUNE_LDS = 1, TUNE_GRAPH = 2, TUNE_FUSED = 4, TUNE_WIDE = 8 };
+enum { TUNE_LDS = 1, TUNE_GRAPH = 2, TUNE_FUSED = 4, TUNE_WIDE = 8, TUNE_CKPT = 16 };
 static int tune_flags(const Tune &t) {
     return (t.lds ? TUNE_LDS : 0) | (t.graph ? TUNE_GRAPH : 0) | (t.fused ? TUNE_FUSED : 0) |
-           (t.wide ? TUNE_WIDE : 0);
+           (t.wide ? TUNE_WIDE : 0) | (t.ckpt ? TUNE_CKPT : 0);
 }
 
 static int pow2_at_least(int64_t n) {
@@ -773,17 +775,21 @@ static int ensure_rev(DevPlan &pl) {
     pl.nred = (int64_t)rp.red.size();
     pl.lvl_seg = rp.lvl_seg;
     pl.nseg = (int64_t)rp.segs.size();
+    grad_levels_full(lay, &pl.gl_full);
+    grad_levels_checkpointed(lay, &pl.gl_ckpt);
     pl.rev_bytes = acc;
     pl.rev_ready = true;
     return EGGMAN_OK;
 }
 
-// Device bytes one instance needs for forward + reverse: every level of H, two levels of G,
-// the dP partials (segment sums, or task rows plus stage-one rows), and its P and dP columns.
-static size_t grad_bytes_per_instance(const DevPlan &pl, int dtype) {
+// Device bytes one instance needs for forward + reverse: the kept levels of H and the segment
+// region (every level, without checkpoints), two levels of G, the dP partials (segment sums, or
+// task rows plus stage-one rows), and its P and dP columns.
+static size_t grad_bytes_per_instance(const DevPlan &pl, int dtype, bool ckpt) {
     const size_t hb = elt_hbytes(dtype), pb = elt_pbytes(dtype);
     const PlanLayout &lay = pl.lay;
-    return (size_t)(lay.nstates + 2 * lay.max_level) * hb +
+    const GradLevels &gl = ckpt ? pl.gl_ckpt : pl.gl_full;
+    return (size_t)(gl.kept_rows + gl.seg_rows + 2 * lay.max_level) * hb +
            (size_t)std::max<int64_t>(std::max<int64_t>(pl.nseg, pl.part_rows + pl.nred), 1) * hb +
            (size_t)lay.npairs * (pb + hb);
 }
@@ -903,7 +909,10 @@ static void launch_rev_g(const DevCaps &caps, const Tune &tune, const DevPlan &p
         (typename E::H *)Gc, nb, map);
 }
 
-// Forward walk keeping every level, then the reverse sweep: out[b] = haf, dP[q][b] = d haf/d P.
+// Forward walk, then the reverse sweep: out[b] = haf, dP[q][b] = d haf/d P.  The forward keeps
+// every level, or (tune.ckpt) only the checkpoint levels of pl.gl_ckpt; a level that was not
+// kept is recomputed with the rest of its segment from the kept level below when the reverse
+// sweep first needs it.  Recomputed values are the same bits (same kernel, same inputs).
 template <int DT, typename W>
 static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t ldp, int64_t B,
                                void *out, void *dP, int64_t lddp, hipStream_t st,
@@ -915,7 +924,8 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
     const size_t hb = sizeof(HT), pb = sizeof(typename E::P);
     const int64_t chunk = tune.chunk > 0 ? std::min(tune.chunk, B) : B;
     const bool par = sizeof(W) == 4 && use_par_dp(ds.caps, pl, DT);
-    EGG_TRY(ensure(ds.Hg, (size_t)lay.nstates * chunk * hb));
+    const GradLevels &gl = tune.ckpt ? pl.gl_ckpt : pl.gl_full;
+    EGG_TRY(ensure(ds.Hg, (size_t)std::max<int64_t>(gl.kept_rows + gl.seg_rows, 1) * chunk * hb));
     EGG_TRY(ensure(ds.G, (size_t)2 * lay.max_level * chunk * hb));
     EGG_TRY(ensure(ds.part, (size_t)std::max<int64_t>(par ? pl.part_rows + pl.nred : pl.nseg, 1) * chunk * hb));
     HT *H = (HT *)ds.Hg.p;
@@ -927,18 +937,35 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
         const char *Pc = (const char *)P + (size_t)b0 * pb;
         HT *outc = (HT *)out + b0;
         HT *dPc = (HT *)dP + b0;
-        auto level_rows = [&](int L) { return H + (size_t)(lay.levels[L - 1] - 1) * nb; };
-        k_fill_one<DT><<<dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st>>>(level_rows(1), nb);
-        for (int L = 2; L <= top; L++) {
+        // rows of level L (1 .. top-1) in the kept region or the segment region
+        auto level_rows = [&](int L) {
+            return H + (size_t)((gl.kept[L] ? 0 : gl.kept_rows) + gl.off[L]) * nb;
+        };
+        auto forward_level = [&](int L) {
             const void *Hc = level_rows(L - 1);
             void *Ho = L == top ? (void *)outc : (void *)level_rows(L);
             if (!(tune.lds && launch_level_lds<DT, W, 1>(ds.caps, tune, pl, L, Pc, ldp, Hc, nb, Ho, nb, nb, st)))
                 launch_level_bt<DT, W>(64, pl, L, Pc, ldp, Hc, nb, Ho, nb, nb, st);
+        };
+        k_fill_one<DT><<<dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st>>>(level_rows(1), nb);
+        for (int L = 2; L <= top; L++) forward_level(L);
+        // the segment the forward left in the segment region: the run of levels below top
+        int seg_lo = top, seg_hi = top - 1;
+        if (!gl.kept[top - 1]) {
+            seg_lo = top - 1;
+            while (!gl.kept[seg_lo - 1]) seg_lo--;
         }
         // reverse: G of the full set is one
         k_fill_one<DT><<<dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st>>>(G[top & 1], nb);
         const int64_t ntiles = (nb + 63) / 64;
         for (int L = top; L >= 2; L--) {
+            if (!gl.kept[L - 1] && !(seg_lo <= L - 1 && L - 1 <= seg_hi)) {
+                // level L-1 heads a segment that is not resident: recompute the segment
+                seg_hi = L - 1;
+                seg_lo = L - 1;
+                while (!gl.kept[seg_lo - 1]) seg_lo--;
+                for (int l = seg_lo; l <= seg_hi; l++) forward_level(l);
+            }
             if (par) {
                 launch_dp_par<DT>(ds.caps, pl, L, level_rows(L - 1), G[L & 1], part, nb, st);
             } else {
@@ -1004,17 +1031,42 @@ static int64_t grad_pass(const Tune &t, int64_t B, int64_t cap) {
     return std::max<int64_t>(1, std::min(std::min(c, B), cap));
 }
 
-// Largest chunk of instances whose gradient buffers fit in `frac` of the free device memory.
-static int grad_chunk(const DevPlan &pl, int dtype, int64_t want, int64_t *chunk) {
+// Instances whose gradient buffers fit in 80% of the free device memory (0 if not one does).
+static int grad_fit(const DevPlan &pl, int dtype, bool ckpt, int64_t *fit) {
     size_t free_b = 0, total_b = 0;
     EGG_HIP(hipMemGetInfo(&free_b, &total_b));
-    const size_t per = grad_bytes_per_instance(pl, dtype);
-    int64_t fit = (int64_t)((double)free_b * 0.8 / (double)per);
-    if (fit >= 64) fit -= fit % 64;
+    const size_t per = grad_bytes_per_instance(pl, dtype, ckpt);
+    int64_t f = (int64_t)((double)free_b * 0.8 / (double)per);
+    if (f >= 64) f -= f % 64;
+    *fit = std::max<int64_t>(f, 0);
+    return EGGMAN_OK;
+}
+
+// The gradient's launch choices for up to `want` instances: the forward's tuned choices
+// (waves per workgroup, LDS staging); checkpointed levels when the tuning entry asks for them,
+// or when keeping every level would hold a pass below min(want, kGradMinPass) instances and
+// checkpoints allow a larger one; `cap` = the instances one call may hold in memory, and the
+// pass size within it.
+static int grad_tune(const DevPlan &pl, int dtype, int64_t want, Tune *tune, int64_t *cap) {
+    Tune t = pick_tune(pl, dtype, std::max<int64_t>(want, 1));
+    int64_t fit;
+    EGG_TRY(grad_fit(pl, dtype, t.ckpt, &fit));
+    if (!t.ckpt && fit < std::min(want, kGradMinPass)) {
+        int64_t fit_ck;
+        EGG_TRY(grad_fit(pl, dtype, true, &fit_ck));
+        if (fit_ck > fit) {
+            t.ckpt = true;
+            fit = fit_ck;
+        }
+    }
     if (fit < 1)
-        return fail(EGGMAN_ENOMEM, "one instance of the degree-%d gradient needs %.2f GB; %.2f GB free",
-                    pl.lay.K, per / 1e9, free_b / 1e9);
-    *chunk = std::min(want, fit);
+        return fail(EGGMAN_ENOMEM, "one instance of the degree-%d gradient needs %.2f GB (%.2f GB "
+                    "with checkpointed levels); not enough free device memory", pl.lay.K,
+                    grad_bytes_per_instance(pl, dtype, false) / 1e9,
+                    grad_bytes_per_instance(pl, dtype, true) / 1e9);
+    *cap = std::min(want, fit);
+    t.chunk = grad_pass(t, want, *cap);
+    *tune = t;
     return EGGMAN_OK;
 }
 
@@ -1398,7 +1450,7 @@ int eggman_tune_set(int K, int index_bytes, int dtype, int64_t B, int bt, int64_
                     int flags, int occ) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     if (!elt_hbytes(dtype) || B < 1 || (bt & (bt - 1)) || bt < 1 || bt > 64 || occ < 0 ||
-        (flags & ~(TUNE_LDS | TUNE_GRAPH | TUNE_FUSED | TUNE_WIDE)))
+        (flags & ~(TUNE_LDS | TUNE_GRAPH | TUNE_FUSED | TUNE_WIDE | TUNE_CKPT)))
         return fail(EGGMAN_EINVAL, "bad tuning entry");
     int dev;
     EGG_TRY(current_device(&dev));
@@ -1410,6 +1462,7 @@ int eggman_tune_set(int K, int index_bytes, int dtype, int64_t B, int bt, int64_
     t.graph = (flags & TUNE_GRAPH) != 0;
     t.fused = (flags & TUNE_FUSED) != 0;
     t.wide = (flags & TUNE_WIDE) != 0;
+    t.ckpt = (flags & TUNE_CKPT) != 0;
     g_tune[TuneKey(dev, K, index_bytes, dtype, batch_bucket(B))] = t;
     return EGGMAN_OK;
 }
@@ -1533,14 +1586,12 @@ int eggman_dp_forward_grad(uint64_t plan, int dtype, const void *P_host, int64_t
     EGG_TRY(dev_state(pl->device, &ds));
     EGG_TRY(chain_stream(*ds, ds->stream));
     EGG_TRY(ensure_rev(*pl));
-    int64_t chunk = max_batch > 0 ? std::min(max_batch, B) : B;
-    EGG_TRY(grad_chunk(*pl, dtype, chunk, &chunk));
+    Tune tune;
+    int64_t chunk;
+    EGG_TRY(grad_tune(*pl, dtype, max_batch > 0 ? std::min(max_batch, B) : B, &tune, &chunk));
     EGG_TRY(ensure(ds->P, (size_t)chunk * npairs * pb));
     EGG_TRY(ensure(ds->dP, (size_t)chunk * npairs * hb));
     EGG_TRY(ensure(ds->out, (size_t)chunk * hb));
-    // the forward's tuned choices (waves per workgroup, LDS staging) serve the gradient
-    Tune tune = pick_tune(*pl, dtype, chunk);
-    tune.chunk = grad_pass(tune, chunk, chunk);
     std::vector<char> stage, gstage;
     for (int64_t b0 = 0; b0 < B; b0 += chunk) {
         const int64_t nb = std::min(chunk, B - b0);
@@ -1594,10 +1645,9 @@ int eggman_dp_forward_grad_device(uint64_t plan, int dtype, const void *P_dev, i
     hipStream_t st = stream ? (hipStream_t)stream : ds->stream;
     EGG_TRY(chain_stream(*ds, st));
     EGG_TRY(ensure_rev(*pl));
-    int64_t chunk = B;
-    EGG_TRY(grad_chunk(*pl, dtype, B, &chunk));
-    Tune tune = pick_tune(*pl, dtype, chunk);
-    tune.chunk = grad_pass(tune, B, chunk);
+    Tune tune;
+    int64_t cap;
+    EGG_TRY(grad_tune(*pl, dtype, B, &tune, &cap));
     return forward_grad_any(dtype, *ds, *pl, P_dev, ldp, B, out_dev, dP_dev, lddp, st, tune);
 }
 
@@ -1646,6 +1696,24 @@ int64_t eggman_batch_bytes(int K, int dtype, int64_t B) {
     int rc = plan_layout(K, 8, &lay);
     if (rc != EGGMAN_OK && rc != EGGMAN_EPLAN) return rc;
     return B * (lay.nstates + lay.npairs) * (int64_t)hb;
+}
+
+int eggman_grad_levels(int K, int checkpointed, int64_t *kept_rows, int64_t *seg_rows,
+                       uint8_t *kept, int64_t nkept) {
+    PlanLayout lay;
+    int rc = plan_layout(K, 8, &lay);
+    if (rc != EGGMAN_OK && rc != EGGMAN_EPLAN) return rc;
+    GradLevels gl;
+    if (checkpointed) grad_levels_checkpointed(lay, &gl);
+    else grad_levels_full(lay, &gl);
+    if (kept_rows) *kept_rows = gl.kept_rows;
+    if (seg_rows) *seg_rows = gl.seg_rows;
+    if (kept) {
+        if (nkept < (int64_t)gl.kept.size())
+            return fail(EGGMAN_EINVAL, "kept needs %lld entries", (long long)gl.kept.size());
+        for (size_t L = 0; L < gl.kept.size(); L++) kept[L] = L == 0 ? 0 : (uint8_t)gl.kept[L];
+    }
+    return EGGMAN_OK;
 }
 
 }  // extern "C"

@@ -280,6 +280,83 @@ int build_rev_plan(const PlanLayout &lay, const void *trans, int seg_len, int ta
     return build_rev_t<int64_t>(lay, (const int64_t *)trans, seg_len, task_len, fan_in, rp);
 }
 
+void grad_levels_full(const PlanLayout &lay, GradLevels *g) {
+    const int n = lay.K / 2;   // levels 1 .. n hold values the reverse sweep reads
+    g->kept.assign(n + 1, 1);
+    g->off.assign(n + 1, 0);
+    g->kept_rows = 0;
+    g->seg_rows = 0;
+    for (int L = 1; L <= n; L++) {
+        g->off[L] = g->kept_rows;
+        g->kept_rows += lay.nst[L - 1];
+    }
+}
+
+void grad_levels_checkpointed(const PlanLayout &lay, GradLevels *g) {
+    const int n = lay.K / 2;
+    std::vector<int64_t> S(n + 1, 0);   // prefix sums of level rows
+    for (int L = 1; L <= n; L++) S[L] = S[L - 1] + lay.nst[L - 1];
+    auto run = [&](int a, int b) { return a > b ? (int64_t)0 : S[b] - S[a - 1]; };
+    std::vector<int64_t> bounds{0};
+    for (int a = 2; a <= n; a++)
+        for (int b = a; b <= n; b++) bounds.push_back(run(a, b));
+    std::sort(bounds.begin(), bounds.end());
+    bounds.erase(std::unique(bounds.begin(), bounds.end()), bounds.end());
+    const int64_t INF = INT64_MAX / 4;
+    int64_t best = INF;
+    std::vector<char> best_kept;
+    std::vector<int64_t> f(n + 1), prev(n + 1);
+    for (int64_t T : bounds) {
+        if (T >= best) break;
+        // f[j]: fewest kept rows for levels 1..j with j kept and every segment below j <= T
+        f[1] = lay.nst[0];
+        prev[1] = 0;
+        for (int j = 2; j <= n; j++) {
+            f[j] = INF;
+            for (int i = 1; i < j; i++)
+                if (f[i] < INF && run(i + 1, j - 1) <= T && f[i] + lay.nst[j - 1] < f[j]) {
+                    f[j] = f[i] + lay.nst[j - 1];
+                    prev[j] = i;
+                }
+        }
+        int last = 0;
+        int64_t kept = INF;
+        for (int i = 1; i <= n; i++)
+            if (f[i] < kept && run(i + 1, n) <= T) {
+                kept = f[i];
+                last = i;
+            }
+        if (kept >= INF) continue;
+        std::vector<char> k(n + 1, 0);
+        for (int i = last; i >= 1; i = (int)prev[i]) k[i] = 1;
+        int64_t seg = 0, cur = 0;
+        for (int L = 2; L <= n; L++) {
+            cur = k[L] ? 0 : cur + lay.nst[L - 1];
+            seg = std::max(seg, cur);
+        }
+        if (kept + seg < best) {
+            best = kept + seg;
+            best_kept = k;
+        }
+    }
+    g->kept = best_kept;
+    g->off.assign(n + 1, 0);
+    g->kept_rows = 0;
+    g->seg_rows = 0;
+    int64_t cur = 0;
+    for (int L = 1; L <= n; L++) {
+        if (g->kept[L]) {
+            g->off[L] = g->kept_rows;
+            g->kept_rows += lay.nst[L - 1];
+            cur = 0;
+        } else {
+            g->off[L] = cur;
+            cur += lay.nst[L - 1];
+            g->seg_rows = std::max(g->seg_rows, cur);
+        }
+    }
+}
+
 // Check that caller-supplied arrays are a plan the kernels can walk safely: the closed-form
 // sizes and level boundaries, a uniform out-degree per level (what lets the device drop
 // `starts`), and every transition pointing into the level below with a valid pair index.

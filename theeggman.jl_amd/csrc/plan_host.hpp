@@ -82,6 +82,24 @@ struct RevPlanHost {
 int build_rev_plan(const PlanLayout &lay, const void *trans, int seg_len, int task_len,
                    int fan_in, RevPlanHost *rp);
 
+// Where the gradient keeps the forward values of levels 1 .. K/2 (level L's values are read by
+// the reverse step at L + 1; the top level's value is the result).  A kept level stays resident
+// in the kept region at row `off[L]`; the other levels form segments (maximal runs of levels
+// that are not kept) that share one segment region, each starting at row 0, and are recomputed
+// from the kept level below when the reverse sweep reaches them.  Rows are state rows of one
+// pass's instances.
+struct GradLevels {
+    std::vector<char> kept;       // index L
+    std::vector<int64_t> off;     // index L
+    int64_t kept_rows = 0, seg_rows = 0;
+};
+
+// Every level kept.
+void grad_levels_full(const PlanLayout &lay, GradLevels *g);
+// Level 1 and a set of checkpoint levels kept, chosen to minimise kept rows + the largest
+// segment (exact, by a shortest-path search for every candidate segment bound).
+void grad_levels_checkpointed(const PlanLayout &lay, GradLevels *g);
+
 const Binom &host_binom();
 int pair_bits_for(int index_bytes);
 int plan_layout(int K, int index_bytes, PlanLayout *lay);

@@ -12,7 +12,7 @@ from .plan import (DP_MAX, DP_HARD_MAX, UNROLL_MAX, HafnianDPPlan, PlanRef, _bui
 from .backend import (HIPBackend, _device_plan, _haf_dp_backend, _haf_dp_backend_grad,
                       _haf_dp_backend_mixed,
                       _haf_dp_device,
-                      dp_batch_bytes, empty_gpu_cache, gpu_cache_bytes)
+                      dp_batch_bytes, empty_gpu_cache, gpu_cache_bytes, grad_levels)
 from .hafnian import (_gather_pairs_batch, default_backend, hafnian, hafnian_exact,
                       hafnian_grad, hafnian_mixed, hafnian_repeated, hafnian_repeated_grad,
                       reduction, rrule)

@@ -249,3 +249,16 @@ def dp_batch_bytes(N: int, T, B: int) -> int:
     (src/dp.jl:584-587)."""
     nstates, _ = _dp_counts(N)
     return B * (nstates + N * (N - 1) // 2) * np.dtype(T).itemsize
+
+
+def grad_levels(N: int, checkpointed: bool = True):
+    """Which forward levels the gradient keeps resident for degree N: ``(kept_rows, seg_rows,
+    kept)`` with ``kept[L]`` true for level L (1..N/2) kept, the others recomputed segment by
+    segment in ``seg_rows`` state rows (rows per instance).  New: the reference has no
+    derivative."""
+    import ctypes
+    kr, sr = ctypes.c_int64(), ctypes.c_int64()
+    mask = (ctypes.c_uint8 * (N // 2 + 1))()
+    check(lib().eggman_grad_levels(N, 1 if checkpointed else 0, ctypes.byref(kr), ctypes.byref(sr),
+                                   ctypes.cast(mask, ctypes.c_void_p), len(mask)), "grad_levels")
+    return kr.value, sr.value, [bool(mask[L]) for L in range(len(mask))]
