@@ -262,7 +262,9 @@ def main():
     # level launches per step: K/2 levels for every pass of `chunk` instances
     chunk = (tuned or {}).get("chunk") or B
     launches = (K // 2) * (-(-B // min(chunk, B)))
-    head = "k_level_fwd_lds" if (tuned or {}).get("lds", 1) and not (tuned or {}).get("fused") else "k_level_fwd"
+    tl = tuned or {}
+    head = ("k_level_fwd_lds_narrow" if tl.get("narrow") else "k_level_fwd_lds") \
+        if tl.get("lds", 1) and not tl.get("fused") else "k_level_fwd"
     achieved = nbytes / (ms / 1e3) / 1e9
     result = {
         "metric": "DP transitions/s (batched N=28 FP64 forward)",

@@ -202,6 +202,8 @@ int eggman_autotune(uint64_t plan, int dtype, int64_t B, int reps, char *report,
 #define EGGMAN_TUNE_CKPT 16 /* gradient: keep checkpoint levels only and recompute the others
                                (chosen by itself when keeping every level would not fit a
                                full pass in device memory) */
+#define EGGMAN_TUNE_NARROW 32 /* LDS kernel with 16-instance tiles and 4 states per wave: a
+                                 child row of a tile is one cache line, so L2 holds 4x more */
 
 /*
  * Set / read a tuning entry directly (e.g. to persist tuned choices across processes).  bt:

@@ -86,7 +86,7 @@ struct GraphEntry {
     int64_t chunk = 0;
     bool lds = false;
     int occ = 0;
-    bool fused = false, wide = false;
+    bool fused = false, wide = false, narrow = false;
     uint64_t gen = 0;                // work-buffer generation the graph's addresses belong to
     hipGraphExec_t exec = nullptr;
 };
@@ -207,19 +207,21 @@ struct DeviceGuard {
 struct Tune {
     int bt = 64;        // lanes per state (batch tile width), power of two in [1, 64]
     int64_t chunk = 0;  // instances per pass over the levels (0 = all)
+    int64_t fchunk = 0; // gradient: instances per forward sub-pass inside a pass (0 = the pass)
     bool lds = true;    // stage the level's pair rows in LDS (bt == 64 only)
     int occ = 0;        // workgroups per CU for the LDS kernel (0 = as many as LDS allows, <= 4)
     bool graph = false; // replay the level sequence from a captured HIP graph
     bool fused = false; // one workgroup per instance walks every level with H in LDS (small K)
     bool wide = false;  // 16 waves per workgroup in the LDS kernel instead of 8
     bool ckpt = false;  // gradient: keep checkpoint levels only and recompute the others
+    bool narrow = false;// LDS kernel with 16-instance tiles, 4 states per wave
 };
 
 // Tuning flags as they cross the C ABI (eggman_tune_set / eggman_tune_get).
-enum { TUNE_LDS = 1, TUNE_GRAPH = 2, TUNE_FUSED = 4, TUNE_WIDE = 8, TUNE_CKPT = 16 };
+enum { TUNE_LDS = 1, TUNE_GRAPH = 2, TUNE_FUSED = 4, TUNE_WIDE = 8, TUNE_CKPT = 16, TUNE_NARROW = 32 };
 static int tune_flags(const Tune &t) {
     return (t.lds ? TUNE_LDS : 0) | (t.graph ? TUNE_GRAPH : 0) | (t.fused ? TUNE_FUSED : 0) |
-           (t.wide ? TUNE_WIDE : 0) | (t.ckpt ? TUNE_CKPT : 0);
+           (t.wide ? TUNE_WIDE : 0) | (t.ckpt ? TUNE_CKPT : 0) | (t.narrow ? TUNE_NARROW : 0);
 }
 
 static int pow2_at_least(int64_t n) {
@@ -291,24 +293,27 @@ constexpr int kLdsWaves = 8;   // waves per workgroup of the LDS kernels (16 wit
 // Workgroup-to-(tile, states) mapping of the LDS kernels for `nst` states of `nb` instances:
 // enough workgroups per XCD to fill its CUs at the occupancy the LDS use allows (and at most 32
 // waves per CU), and no more than there are states.
+// `tile` instances per tile (64, or 16 for the narrow kernel) and `spw` states per wave step
+// (1, or 4 for the narrow kernel).
 static LdsMap make_lds_map(const DevCaps &caps, int occ_cap, int64_t nst, int64_t nb, size_t lds,
-                           int wpb = kLdsWaves) {
+                           int wpb = kLdsWaves, int tile = 64, int spw = 1) {
     int occ = (int)std::max<size_t>(1, std::min<size_t>(32 / wpb, caps.lds_per_cu / std::max<size_t>(lds, 1)));
     if (occ_cap > 0) occ = std::min(occ, occ_cap);
     const int cus_xcd = std::max(1, caps.cus / caps.xcds);
     const int64_t target = (int64_t)cus_xcd * occ;
+    const int64_t per_wg = (int64_t)wpb * spw;   // states one workgroup step covers
     LdsMap map;
-    map.ntiles = (nb + 63) / 64;
+    map.ntiles = (nb + tile - 1) / tile;
     if (map.ntiles >= 8) {
         map.tiles_xcd = (map.ntiles + 7) / 8;
         int64_t bpt = std::max<int64_t>(1, target / map.tiles_xcd);
-        bpt = std::min<int64_t>(bpt, std::max<int64_t>(1, (nst + wpb - 1) / wpb));
+        bpt = std::min<int64_t>(bpt, std::max<int64_t>(1, (nst + per_wg - 1) / per_wg));
         map.per_xcd = map.tiles_xcd * bpt;
     } else {
         map.tiles_xcd = 0;
         const int64_t groups = (8 + map.ntiles - 1) / map.ntiles;
         const int64_t per_group = (nst + groups - 1) / groups;
-        map.per_xcd = std::max<int64_t>(1, std::min<int64_t>(target, (per_group + wpb - 1) / wpb));
+        map.per_xcd = std::max<int64_t>(1, std::min<int64_t>(target, (per_group + per_wg - 1) / per_wg));
     }
     return map;
 }
@@ -326,17 +331,18 @@ static void level_pairs(const PlanLayout &lay, int L, int *qlo, int *nq) {
 // Launch one level with the LDS kernel; returns false (launching nothing) when the level's pair
 // rows do not fit in one workgroup's LDS.  TAG only separates the instantiation used by the
 // gradient's forward pass from the plain forward in profiles; the code is the same.
-template <int DT, typename W, int WPB, int TAG>
+template <int DT, typename W, int WPB, int TAG, bool NARROW = false>
 static bool launch_level_lds_w(const DevCaps &caps, const Tune &tune, const DevPlan &pl, int L,
                                const void *P, int64_t ldp, const void *Hc, int64_t ldh, void *Ho,
                                int64_t ldo, int64_t nb, hipStream_t st) {
     using E = Elt<DT>;
     const PlanLayout &lay = pl.lay;
+    constexpr int TW = NARROW ? 16 : 64;
     int qlo, nq;
     level_pairs(lay, L, &qlo, &nq);
-    const size_t lds = (size_t)nq * 64 * sizeof(typename E::P);
+    const size_t lds = (size_t)nq * TW * sizeof(typename E::P);
     if (lds > caps.max_lds_block) return false;
-    auto kern = k_level_fwd_lds<DT, W, WPB, TAG>;
+    auto kern = NARROW ? k_level_fwd_lds_narrow<DT, W, WPB, TAG> : k_level_fwd_lds<DT, W, WPB, TAG>;
     static bool attr_set = false;
     if (!attr_set) {
         if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -347,7 +353,7 @@ static bool launch_level_lds_w(const DevCaps &caps, const Tune &tune, const DevP
         attr_set = true;
     }
     const int64_t nst = lay.nst[L - 1];
-    const LdsMap map = make_lds_map(caps, tune.occ, nst, nb, lds, WPB);
+    const LdsMap map = make_lds_map(caps, tune.occ, nst, nb, lds, WPB, TW, NARROW ? 4 : 1);
     LevelArgs<DT, W> a;
     a.trans_lvl = (const W *)pl.d_trans + lay.tstart[L - 1];
     a.nst = nst;
@@ -371,6 +377,11 @@ template <int DT, typename W, int TAG = 0>
 static bool launch_level_lds(const DevCaps &caps, const Tune &tune, const DevPlan &pl, int L,
                              const void *P, int64_t ldp, const void *Hc, int64_t ldh, void *Ho,
                              int64_t ldo, int64_t nb, hipStream_t st) {
+    if (tune.narrow) {
+        if (tune.wide)
+            return launch_level_lds_w<DT, W, 16, TAG, true>(caps, tune, pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st);
+        return launch_level_lds_w<DT, W, kLdsWaves, TAG, true>(caps, tune, pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st);
+    }
     if (tune.wide)
         return launch_level_lds_w<DT, W, 16, TAG>(caps, tune, pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st);
     return launch_level_lds_w<DT, W, kLdsWaves, TAG>(caps, tune, pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st);
@@ -498,7 +509,8 @@ static int forward_graph(int dtype, DevState &ds, const DevPlan &pl, const void 
         if (it->gen != g_buf_gen) continue;
         if (it->plan == pl.id && it->dtype == dtype && it->P == P && it->ldp == ldp && it->B == B &&
             it->out == out && it->bt == tune.bt && it->chunk == tune.chunk && it->lds == tune.lds &&
-            it->occ == tune.occ && it->fused == tune.fused && it->wide == tune.wide) {
+            it->occ == tune.occ && it->fused == tune.fused && it->wide == tune.wide &&
+            it->narrow == tune.narrow) {
             EGG_HIP(hipGraphLaunch(it->exec, st));
             return EGGMAN_OK;
         }
@@ -528,6 +540,7 @@ static int forward_graph(int dtype, DevState &ds, const DevPlan &pl, const void 
     ge.occ = tune.occ;
     ge.fused = tune.fused;
     ge.wide = tune.wide;
+    ge.narrow = tune.narrow;
     ge.gen = g_buf_gen;
     const hipError_t ei = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
@@ -555,7 +568,7 @@ static int forward_any(int dtype, DevState &ds, const DevPlan &pl, const void *P
 // single-degree walk does) and keeps two-level state buffers of that size; the groups' passes run
 // in rounds, round r walking pass r of every group that has one.  P is the concatenation of the
 // groups' [pair][count] blocks, out of their results.
-template <int DT, typename W, int WPB>
+template <int DT, typename W, int WPB, bool NARROW>
 static int forward_mixed_levels(DevState &ds, const std::vector<DevPlan *> &pls,
                                 const std::vector<int64_t> &counts, const void *P, void *out,
                                 hipStream_t st) {
@@ -587,7 +600,8 @@ static int forward_mixed_levels(DevState &ds, const std::vector<DevPlan *> &pls,
         buf1[g] = buf0[g] + rows;
         hoff += 2 * rows;
     }
-    auto kern = k_level_fwd_lds_multi<DT, W, WPB>;
+    auto kern = k_level_fwd_lds_multi<DT, W, WPB, NARROW>;
+    constexpr int TW = NARROW ? 16 : 64;
     static bool attr_set = false;
     if (!attr_set) {
         attr_set = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -626,7 +640,7 @@ static int forward_mixed_levels(DevState &ds, const std::vector<DevPlan *> &pls,
                 HT *Ho = L == lay.K / 2 + 1 ? outg[g] + b0[g] : (L & 1 ? buf1[g] : buf0[g]);
                 int qlo, nq;
                 level_pairs(lay, L, &qlo, &nq);
-                const size_t lds = (size_t)nq * 64 * sizeof(PT);
+                const size_t lds = (size_t)nq * TW * sizeof(PT);
                 if (!attr_set || !tg[g].lds || lds > ds.caps.max_lds_block) {
                     launch_level_bt<DT, W>(std::min(tg[g].bt, pow2_at_least(nb)), pl, L, Pc, counts[g],
                                            Hc, nb, Ho, nb, nb, st);
@@ -646,7 +660,7 @@ static int forward_mixed_levels(DevState &ds, const std::vector<DevPlan *> &pls,
                 a.Ho = Ho;
                 a.ldo = nb;
                 a.nb = nb;
-                a.map = make_lds_map(ds.caps, tg[g].occ, a.nst, nb, lds, WPB);
+                a.map = make_lds_map(ds.caps, tg[g].occ, a.nst, nb, lds, WPB, TW, NARROW ? 4 : 1);
                 m.first[m.n + 1] = m.first[m.n] + 8 * a.map.per_xcd;
                 m.n++;
                 lds_max = std::max(lds_max, lds);
@@ -659,7 +673,19 @@ static int forward_mixed_levels(DevState &ds, const std::vector<DevPlan *> &pls,
     return EGGMAN_OK;
 }
 
-// The workgroup width follows the tuned choice of the group with the most transitions.
+// The workgroup width and the tile width follow the tuned choice of the group with the most
+// transitions (one kernel serves every group of a launch).
+template <int DT, typename W>
+static int forward_mixed_w(DevState &ds, const std::vector<DevPlan *> &pls,
+                           const std::vector<int64_t> &counts, const void *P, void *out,
+                           hipStream_t st, bool wide, bool narrow) {
+    if (narrow)
+        return wide ? forward_mixed_levels<DT, W, 16, true>(ds, pls, counts, P, out, st)
+                    : forward_mixed_levels<DT, W, kLdsWaves, true>(ds, pls, counts, P, out, st);
+    return wide ? forward_mixed_levels<DT, W, 16, false>(ds, pls, counts, P, out, st)
+                : forward_mixed_levels<DT, W, kLdsWaves, false>(ds, pls, counts, P, out, st);
+}
+
 template <int DT>
 static int forward_mixed_dt(DevState &ds, const std::vector<DevPlan *> &pls,
                             const std::vector<int64_t> &counts, const void *P, void *out,
@@ -667,12 +693,10 @@ static int forward_mixed_dt(DevState &ds, const std::vector<DevPlan *> &pls,
     size_t big = 0;
     for (size_t g = 1; g < pls.size(); g++)
         if ((double)pls[g]->lay.ntrans * counts[g] > (double)pls[big]->lay.ntrans * counts[big]) big = g;
-    const bool wide = pick_tune(*pls[big], DT, std::max<int64_t>(counts[big], 1)).wide;
+    const Tune t = pick_tune(*pls[big], DT, std::max<int64_t>(counts[big], 1));
     if (pls[0]->lay.index_bytes == 4)
-        return wide ? forward_mixed_levels<DT, int32_t, 16>(ds, pls, counts, P, out, st)
-                    : forward_mixed_levels<DT, int32_t, kLdsWaves>(ds, pls, counts, P, out, st);
-    return wide ? forward_mixed_levels<DT, int64_t, 16>(ds, pls, counts, P, out, st)
-                : forward_mixed_levels<DT, int64_t, kLdsWaves>(ds, pls, counts, P, out, st);
+        return forward_mixed_w<DT, int32_t>(ds, pls, counts, P, out, st, t.wide, t.narrow);
+    return forward_mixed_w<DT, int64_t>(ds, pls, counts, P, out, st, t.wide, t.narrow);
 }
 
 static int forward_mixed_any(int dtype, DevState &ds, const std::vector<DevPlan *> &pls,
@@ -941,11 +965,18 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
         auto level_rows = [&](int L) {
             return H + (size_t)((gl.kept[L] ? 0 : gl.kept_rows) + gl.off[L]) * nb;
         };
+        // the forward runs in sub-passes of the forward's tuned size (its best L2 footprint),
+        // columns [c0, c0 + nc) of the pass's level rows
+        const int64_t sub = tune.fchunk > 0 ? std::min(tune.fchunk, nb) : nb;
         auto forward_level = [&](int L) {
-            const void *Hc = level_rows(L - 1);
-            void *Ho = L == top ? (void *)outc : (void *)level_rows(L);
-            if (!(tune.lds && launch_level_lds<DT, W, 1>(ds.caps, tune, pl, L, Pc, ldp, Hc, nb, Ho, nb, nb, st)))
-                launch_level_bt<DT, W>(64, pl, L, Pc, ldp, Hc, nb, Ho, nb, nb, st);
+            for (int64_t c0 = 0; c0 < nb; c0 += sub) {
+                const int64_t nc = std::min(sub, nb - c0);
+                const void *Pcc = (const char *)Pc + (size_t)c0 * pb;
+                const HT *Hc = level_rows(L - 1) + c0;
+                HT *Ho = (L == top ? outc : level_rows(L)) + c0;
+                if (!(tune.lds && launch_level_lds<DT, W, 1>(ds.caps, tune, pl, L, Pcc, ldp, Hc, nb, Ho, nb, nc, st)))
+                    launch_level_bt<DT, W>(64, pl, L, Pcc, ldp, Hc, nb, Ho, nb, nc, st);
+            }
         };
         k_fill_one<DT><<<dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st>>>(level_rows(1), nb);
         for (int L = 2; L <= top; L++) forward_level(L);
@@ -1065,6 +1096,7 @@ static int grad_tune(const DevPlan &pl, int dtype, int64_t want, Tune *tune, int
                     grad_bytes_per_instance(pl, dtype, false) / 1e9,
                     grad_bytes_per_instance(pl, dtype, true) / 1e9);
     *cap = std::min(want, fit);
+    t.fchunk = t.chunk;                 // the forward's own tuned pass
     t.chunk = grad_pass(t, want, *cap);
     *tune = t;
     return EGGMAN_OK;
@@ -1346,7 +1378,7 @@ int eggman_autotune(uint64_t plan, int dtype, int64_t B, int reps, char *report,
     std::vector<Tune> cands;
     if (B >= 64) {
         std::vector<int64_t> chunks;
-        for (int64_t d : {1, 2, 4, 8}) {
+        for (int64_t d : {1, 2, 4, 8, 16}) {
             int64_t c = (B / d + 63) / 64 * 64;
             if (c >= 64 && std::find(chunks.begin(), chunks.end(), c) == chunks.end()) chunks.push_back(c);
         }
@@ -1355,6 +1387,15 @@ int eggman_autotune(uint64_t plan, int dtype, int64_t B, int reps, char *report,
                 Tune t;
                 t.chunk = c;
                 t.occ = occ;
+                cands.push_back(t);
+                t.wide = true;
+                cands.push_back(t);
+            }
+            for (int occ : {1, 2, 4}) {
+                Tune t;
+                t.chunk = c;
+                t.occ = occ;
+                t.narrow = true;
                 cands.push_back(t);
                 t.wide = true;
                 cands.push_back(t);
@@ -1450,7 +1491,7 @@ int eggman_tune_set(int K, int index_bytes, int dtype, int64_t B, int bt, int64_
                     int flags, int occ) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     if (!elt_hbytes(dtype) || B < 1 || (bt & (bt - 1)) || bt < 1 || bt > 64 || occ < 0 ||
-        (flags & ~(TUNE_LDS | TUNE_GRAPH | TUNE_FUSED | TUNE_WIDE | TUNE_CKPT)))
+        (flags & ~(TUNE_LDS | TUNE_GRAPH | TUNE_FUSED | TUNE_WIDE | TUNE_CKPT | TUNE_NARROW)))
         return fail(EGGMAN_EINVAL, "bad tuning entry");
     int dev;
     EGG_TRY(current_device(&dev));
@@ -1463,6 +1504,7 @@ int eggman_tune_set(int K, int index_bytes, int dtype, int64_t B, int bt, int64_
     t.fused = (flags & TUNE_FUSED) != 0;
     t.wide = (flags & TUNE_WIDE) != 0;
     t.ckpt = (flags & TUNE_CKPT) != 0;
+    t.narrow = (flags & TUNE_NARROW) != 0;
     g_tune[TuneKey(dev, K, index_bytes, dtype, batch_bucket(B))] = t;
     return EGGMAN_OK;
 }

@@ -104,7 +104,7 @@ __global__ __launch_bounds__(kBlock) void k_level_fwd(
 // the subset, and consecutive states share most of their children -- that sweep is what lets
 // the L2 serve most child reads.
 struct LdsMap {
-    int64_t ntiles;      // 64-instance tiles in the chunk
+    int64_t ntiles;      // instance tiles in the chunk (64 wide, or 16 for the narrow kernel)
     int64_t per_xcd;     // workgroups per XCD (R)
     int64_t tiles_xcd;   // tiles per XCD when ntiles >= 8 (ceil(ntiles / 8)), else 0
 };
@@ -199,6 +199,104 @@ __device__ inline void level_lds_body(const LevelArgs<DT, W> &a, int64_t j,
     }
 }
 
+// Narrow-tile variant: 16-instance tiles, a wave covering 4 consecutive states x 16 instances
+// (lane 16j + i: state s0 + j, instance i of the tile).  One state's row of a tile is then 16
+// values -- one 128-byte line for 8-byte values -- instead of 64, so an XCD's L2 holds four
+// times as many child rows, and with 8 tiles per pass each XCD owns one tile and walks every
+// state of it.  The level is bound by L2 misses, which this cuts.  Each lane still sums one
+// (state, instance) in plan order: the results are the same bits as the 64-wide kernel's.
+template <int DT, typename W, int WPB>
+__device__ inline void level_lds_narrow_body(const LevelArgs<DT, W> &a, int64_t j,
+                                             typename Elt<DT>::P *lds) {
+    using E = Elt<DT>;
+    using PT = typename E::P;
+    using HT = typename E::H;
+    constexpr int bits = Packing<W>::bits;
+    constexpr W mask = (W)((W(1) << bits) - 1);
+    constexpr int TW = 16;
+    const LdsMap &map = a.map;
+    const int lane = (int)(threadIdx.x & 63);
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int slot = lane >> 4, li = lane & 15;
+    const int64_t xcd = j & 7, r = j >> 3;
+    int64_t tile, s_begin, s_end, bi, bstride;
+    if (map.tiles_xcd > 0) {
+        const int64_t ti = r % map.tiles_xcd;
+        tile = xcd + 8 * ti;
+        bi = r / map.tiles_xcd;
+        bstride = map.per_xcd / map.tiles_xcd;
+        s_begin = 0;
+        s_end = a.nst;
+        if (tile >= map.ntiles) return;
+    } else {
+        tile = xcd % map.ntiles;
+        const int64_t group = xcd / map.ntiles;
+        const int64_t ngroups = (8 - tile + map.ntiles - 1) / map.ntiles;
+        s_begin = group * a.nst / ngroups;
+        s_end = (group + 1) * a.nst / ngroups;
+        bi = r;
+        bstride = map.per_xcd;
+    }
+    const int64_t b = tile * TW + li;
+    const bool live = b < a.nb;
+    const int64_t bl = live ? b : a.nb - 1;   // dead lanes read a valid column, store nothing
+
+    // stage pair rows [qlo, qlo + nq) of this tile: row i at lds[i * TW]
+    for (int e = (int)threadIdx.x; e < a.nq * TW; e += WPB * 64) {
+        const int64_t bb = tile * TW + (e & (TW - 1));
+        lds[e] = bb < a.nb ? a.P[(int64_t)(a.qlo + e / TW) * a.ldp + bb] : PT{};
+    }
+    __syncthreads();
+
+    const int64_t step = bstride * WPB * 4;
+    constexpr int U = 4;
+    // a state's words are read once, lane li holding words li and 16 + li, and handed to the
+    // group's lanes by a shuffle within the 16-lane group (k <= 31 for the Int32 layout; the
+    // Int64 layout, k <= 63, reads four words per lane)
+    constexpr int NW = sizeof(W) == 4 ? 2 : 4;
+    for (int64_t s0 = s_begin + (bi * WPB + wave) * 4; s0 < s_end; s0 += step) {
+        const int64_t s = s0 + slot;
+        const bool ok = s < s_end;
+        const W *tw = a.trans_lvl + (ok ? s : s_end - 1) * (int64_t)a.k;
+        W wv[NW];
+#pragma unroll
+        for (int c = 0; c < NW; c++) wv[c] = tw[min(16 * c + li, a.k - 1)];
+        auto word = [&](int t) -> W {   // t uniform
+            W w = 0;
+#pragma unroll
+            for (int c = 0; c < NW; c++)
+                if ((t >> 4) == c) w = (W)__shfl(wv[c], t & 15, 16);
+            return w;
+        };
+        HT acc = E::zero();
+        int i = 0;
+        for (; i + U <= a.k; i += U) {
+            W t[U];
+            HT h[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) t[u] = word(i + u);
+#pragma unroll
+            for (int u = 0; u < U; u++) h[u] = a.Hc[((int64_t)(t[u] >> bits) - a.cfirst) * a.ldh + bl];
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                acc = E::muladd(acc, lds[((int)(t[u] & mask) - 1 - a.qlo) * TW + li], h[u]);
+        }
+        for (; i < a.k; i++) {
+            const W t = word(i);
+            acc = E::muladd(acc, lds[((int)(t & mask) - 1 - a.qlo) * TW + li],
+                            a.Hc[((int64_t)(t >> bits) - a.cfirst) * a.ldh + bl]);
+        }
+        if (ok && live) a.Ho[s * a.ldo + b] = acc;
+    }
+}
+
+template <int DT, typename W, int WPB, int TAG>
+__global__ __launch_bounds__(WPB * 64) void k_level_fwd_lds_narrow(LevelArgs<DT, W> a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    level_lds_narrow_body<DT, W, WPB>(a, (int64_t)blockIdx.x,
+                                      reinterpret_cast<typename Elt<DT>::P *>(lds_raw));
+}
+
 template <int DT, typename W, int WPB, int TAG>
 __global__ __launch_bounds__(WPB * 64) void k_level_fwd_lds(LevelArgs<DT, W> a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -218,14 +316,18 @@ struct MultiLevelArgs {
     LevelArgs<DT, W> g[kMaxGroups];
 };
 
-template <int DT, typename W, int WPB>
+template <int DT, typename W, int WPB, bool NARROW>
 __global__ __launch_bounds__(WPB * 64) void k_level_fwd_lds_multi(MultiLevelArgs<DT, W> m) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int64_t j = blockIdx.x;
     int g = 0;
     while (g + 1 < m.n && j >= m.first[g + 1]) g++;
-    level_lds_body<DT, W, WPB>(m.g[g], j - m.first[g],
-                               reinterpret_cast<typename Elt<DT>::P *>(lds_raw));
+    if constexpr (NARROW)
+        level_lds_narrow_body<DT, W, WPB>(m.g[g], j - m.first[g],
+                                          reinterpret_cast<typename Elt<DT>::P *>(lds_raw));
+    else
+        level_lds_body<DT, W, WPB>(m.g[g], j - m.first[g],
+                                   reinterpret_cast<typename Elt<DT>::P *>(lds_raw));
 }
 
 // ---------------------------------------------------------------------------------------
