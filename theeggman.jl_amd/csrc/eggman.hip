@@ -207,7 +207,6 @@ struct DeviceGuard {
 struct Tune {
     int bt = 64;        // lanes per state (batch tile width), power of two in [1, 64]
     int64_t chunk = 0;  // instances per pass over the levels (0 = all)
-    int64_t fchunk = 0; // gradient: instances per forward sub-pass inside a pass (0 = the pass)
     bool lds = true;    // stage the level's pair rows in LDS (bt == 64 only)
     int occ = 0;        // workgroups per CU for the LDS kernel (0 = as many as LDS allows, <= 4)
     bool graph = false; // replay the level sequence from a captured HIP graph
@@ -965,18 +964,11 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
         auto level_rows = [&](int L) {
             return H + (size_t)((gl.kept[L] ? 0 : gl.kept_rows) + gl.off[L]) * nb;
         };
-        // the forward runs in sub-passes of the forward's tuned size (its best L2 footprint),
-        // columns [c0, c0 + nc) of the pass's level rows
-        const int64_t sub = tune.fchunk > 0 ? std::min(tune.fchunk, nb) : nb;
         auto forward_level = [&](int L) {
-            for (int64_t c0 = 0; c0 < nb; c0 += sub) {
-                const int64_t nc = std::min(sub, nb - c0);
-                const void *Pcc = (const char *)Pc + (size_t)c0 * pb;
-                const HT *Hc = level_rows(L - 1) + c0;
-                HT *Ho = (L == top ? outc : level_rows(L)) + c0;
-                if (!(tune.lds && launch_level_lds<DT, W, 1>(ds.caps, tune, pl, L, Pcc, ldp, Hc, nb, Ho, nb, nc, st)))
-                    launch_level_bt<DT, W>(64, pl, L, Pcc, ldp, Hc, nb, Ho, nb, nc, st);
-            }
+            const void *Hc = level_rows(L - 1);
+            void *Ho = L == top ? (void *)outc : (void *)level_rows(L);
+            if (!(tune.lds && launch_level_lds<DT, W, 1>(ds.caps, tune, pl, L, Pc, ldp, Hc, nb, Ho, nb, nb, st)))
+                launch_level_bt<DT, W>(64, pl, L, Pc, ldp, Hc, nb, Ho, nb, nb, st);
         };
         k_fill_one<DT><<<dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st>>>(level_rows(1), nb);
         for (int L = 2; L <= top; L++) forward_level(L);
@@ -1096,8 +1088,15 @@ static int grad_tune(const DevPlan &pl, int dtype, int64_t want, Tune *tune, int
                     grad_bytes_per_instance(pl, dtype, false) / 1e9,
                     grad_bytes_per_instance(pl, dtype, true) / 1e9);
     *cap = std::min(want, fit);
-    t.fchunk = t.chunk;                 // the forward's own tuned pass
     t.chunk = grad_pass(t, want, *cap);
+    // narrow tiles pay off when a pass is at most 8 tiles of 16 (one per XCD); the gradient's
+    // passes are larger, and there the 64-wide kernel with 16 waves per workgroup, 2 per CU
+    // (its best shape at N = 24..32) measured faster (N=28: 17.7 vs 18.1 ms)
+    if (t.narrow && t.chunk > 128) {
+        t.narrow = false;
+        t.wide = true;
+        t.occ = 2;
+    }
     *tune = t;
     return EGGMAN_OK;
 }
