@@ -37,7 +37,10 @@ def test_autotune_choice_is_bit_exact(K, B):
                                      dict(bt=1, chunk=0, lds=0, occ=0, graph=1, fused=1),
                                      dict(bt=64, chunk=64, lds=1, occ=2, narrow=1),
                                      dict(bt=64, chunk=0, lds=1, occ=4, wide=1, narrow=1),
-                                     dict(bt=64, chunk=0, lds=1, occ=2, graph=1, narrow=1)])
+                                     dict(bt=64, chunk=0, lds=1, occ=2, graph=1, narrow=1),
+                                     dict(bt=64, chunk=64, lds=1, occ=2, vec=1),
+                                     dict(bt=64, chunk=0, lds=1, occ=4, wide=1, vec=1),
+                                     dict(bt=64, chunk=0, lds=1, occ=2, graph=1, vec=1)])
 def test_every_launch_choice_gives_the_same_bits(setting):
     K, B = 18, 150
     te.tune_set(K, np.float64, B, **setting)
@@ -77,7 +80,7 @@ def test_tuning_table_round_trip(tmp_path):
     assert te.load_tuning(str(path)) >= 1
     assert te.tune_get(22, np.float64, 512) == {"bt": 64, "chunk": 256, "flags": 1 | 8, "occ": 2,
                                                 "lds": 1, "graph": 0, "fused": 0, "wide": 1,
-                                                "ckpt": 0, "narrow": 0}
+                                                "ckpt": 0, "narrow": 0, "vec": 0}
 
 
 @pytest.mark.parametrize("K,B,T", [(28, 1024, np.float64), (24, 37, np.float64), (22, 130, np.complex128),
@@ -112,3 +115,37 @@ def test_narrow_tiles_give_the_same_bits(K, B, T):
         te.tune_set(K, T, B, bt=64, chunk=0, occ=2, lds=True)
         _, g_wide = te._haf_dp_backend_grad(BE, P, plan)
         assert np.asarray(g_narrow).tobytes() == np.asarray(g_wide).tobytes()
+
+
+@pytest.mark.parametrize("K,B,T", [(28, 1024, np.float64), (24, 37, np.float64), (22, 130, np.complex128),
+                                   (20, 100, np.float32), (16, 70, np.int64), (12, 3, np.complex64),
+                                   ("20/Int64", 300, np.float64)])
+@pytest.mark.parametrize("chunk", [0, 48])
+def test_vector_loads_give_the_same_bits(K, B, T, chunk):
+    I = None
+    if isinstance(K, str):                  # the Int64 plan layout at a small degree
+        K, I = int(K.split("/")[0]), np.int64
+    # 16-byte loads (two or more instances per lane): ragged last tiles and passes, padded level
+    # rows, every element type and both plan layouts, against the 64-wide kernel and the oracle
+    print(f"vec K={K} B={B} {np.dtype(T)} chunk={chunk}", flush=True)
+    if T == np.int64:
+        As = np.stack([synth.gnp_adjacency(K, 67, b) for b in range(B)]).astype(np.int64)
+    elif T in (np.complex128, np.complex64):
+        As = np.stack([synth.sym_complex_normal(K, 67, b) for b in range(B)]).astype(T)
+    else:
+        As = np.stack([synth.sym_uniform(K, 67, b) for b in range(B)]).astype(T)
+    P = te._gather_pairs_batch(T, As, None, K)
+    plan = te._dp_plan(K, I)
+    ib = np.dtype(plan.trans.dtype).itemsize
+    te.tune_set(K, T, B, bt=64, chunk=0, occ=2, lds=True, index_bytes=ib)
+    wide = te._haf_dp_backend(BE, P, plan)
+    te.tune_set(K, T, B, bt=64, chunk=chunk, occ=2, lds=True, vec=True, index_bytes=ib)
+    got = te._haf_dp_backend(BE, P, plan)
+    if T == np.int64:
+        assert list(got) == list(wide)
+    else:
+        assert np.asarray(got).tobytes() == np.asarray(wide).tobytes()
+        if K <= 24:
+            code = {np.float64: o.F64, np.float32: o.F32, np.complex128: o.C64, np.complex64: o.C32}[T]
+            want = o.haf_dp(synth.pair_batch(As), K, code, 8)
+            assert np.asarray(got).tobytes() == want.tobytes()

@@ -84,9 +84,8 @@ struct GraphEntry {
     void *out = nullptr;
     int bt = 0;
     int64_t chunk = 0;
-    bool lds = false;
+    int flags = 0;                   // tune_flags of the launch choices
     int occ = 0;
-    bool fused = false, wide = false, narrow = false;
     uint64_t gen = 0;                // work-buffer generation the graph's addresses belong to
     hipGraphExec_t exec = nullptr;
 };
@@ -214,13 +213,33 @@ struct Tune {
     bool wide = false;  // 16 waves per workgroup in the LDS kernel instead of 8
     bool ckpt = false;  // gradient: keep checkpoint levels only and recompute the others
     bool narrow = false;// LDS kernel with 16-instance tiles, 4 states per wave
+    bool vec = false;   // LDS kernel with 16-byte loads: 8 lanes per state, 8 states per wave
 };
 
 // Tuning flags as they cross the C ABI (eggman_tune_set / eggman_tune_get).
-enum { TUNE_LDS = 1, TUNE_GRAPH = 2, TUNE_FUSED = 4, TUNE_WIDE = 8, TUNE_CKPT = 16, TUNE_NARROW = 32 };
+enum {
+    TUNE_LDS = 1, TUNE_GRAPH = 2, TUNE_FUSED = 4, TUNE_WIDE = 8, TUNE_CKPT = 16, TUNE_NARROW = 32,
+    TUNE_VEC = 64,
+    TUNE_ALL = 127
+};
 static int tune_flags(const Tune &t) {
     return (t.lds ? TUNE_LDS : 0) | (t.graph ? TUNE_GRAPH : 0) | (t.fused ? TUNE_FUSED : 0) |
-           (t.wide ? TUNE_WIDE : 0) | (t.ckpt ? TUNE_CKPT : 0) | (t.narrow ? TUNE_NARROW : 0);
+           (t.wide ? TUNE_WIDE : 0) | (t.ckpt ? TUNE_CKPT : 0) | (t.narrow ? TUNE_NARROW : 0) |
+           (t.vec ? TUNE_VEC : 0);
+}
+static Tune tune_from_flags(int bt, int64_t chunk, int flags, int occ) {
+    Tune t;
+    t.bt = bt;
+    t.chunk = chunk;
+    t.occ = occ;
+    t.lds = (flags & TUNE_LDS) != 0;
+    t.graph = (flags & TUNE_GRAPH) != 0;
+    t.fused = (flags & TUNE_FUSED) != 0;
+    t.wide = (flags & TUNE_WIDE) != 0;
+    t.ckpt = (flags & TUNE_CKPT) != 0;
+    t.narrow = (flags & TUNE_NARROW) != 0;
+    t.vec = (flags & TUNE_VEC) != 0;
+    return t;
 }
 
 static int pow2_at_least(int64_t n) {
@@ -327,21 +346,38 @@ static void level_pairs(const PlanLayout &lay, int L, int *qlo, int *nq) {
     *nq = (int)pair_index(phi, K, K) - 1 - *qlo + 1;
 }
 
-// Launch one level with the LDS kernel; returns false (launching nothing) when the level's pair
+// The LDS level kernels: 64-instance rows, one state per wave (KIND_WIDE); 16-instance tiles,
+// 4 states per wave (KIND_NARROW); 16-byte loads, 8 states per wave (KIND_VEC).
+enum { KIND_WIDE = 0, KIND_NARROW = 1, KIND_VEC = 2 };
+
+template <int DT> constexpr int kind_tile(int kind) {
+    return kind == KIND_VEC ? 128 / (int)sizeof(typename Elt<DT>::H) : kind == KIND_NARROW ? 16 : 64;
+}
+
+// The 16-byte-load kernel reads whole 128-byte tile rows: it needs rows a multiple of its tile
+// apart and 16-byte aligned, with readable padding past the batch.
+template <int DT>
+static bool vec_rows_ok(const void *Hc, int64_t ldh) {
+    return ldh % kind_tile<DT>(KIND_VEC) == 0 && ((uintptr_t)Hc & 15) == 0;
+}
+
+// Launch one level with an LDS kernel; returns false (launching nothing) when the level's pair
 // rows do not fit in one workgroup's LDS.  TAG only separates the instantiation used by the
 // gradient's forward pass from the plain forward in profiles; the code is the same.
-template <int DT, typename W, int WPB, int TAG, bool NARROW = false>
+template <int DT, typename W, int WPB, int TAG, int KIND = KIND_WIDE>
 static bool launch_level_lds_w(const DevCaps &caps, const Tune &tune, const DevPlan &pl, int L,
                                const void *P, int64_t ldp, const void *Hc, int64_t ldh, void *Ho,
                                int64_t ldo, int64_t nb, hipStream_t st) {
     using E = Elt<DT>;
     const PlanLayout &lay = pl.lay;
-    constexpr int TW = NARROW ? 16 : 64;
+    constexpr int TW = kind_tile<DT>(KIND);
     int qlo, nq;
     level_pairs(lay, L, &qlo, &nq);
     const size_t lds = (size_t)nq * TW * sizeof(typename E::P);
     if (lds > caps.max_lds_block) return false;
-    auto kern = NARROW ? k_level_fwd_lds_narrow<DT, W, WPB, TAG> : k_level_fwd_lds<DT, W, WPB, TAG>;
+    auto kern = KIND == KIND_VEC      ? k_level_fwd_lds_vec<DT, W, WPB, TAG>
+                : KIND == KIND_NARROW ? k_level_fwd_lds_narrow<DT, W, WPB, TAG>
+                                      : k_level_fwd_lds<DT, W, WPB, TAG>;
     static bool attr_set = false;
     if (!attr_set) {
         if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -352,7 +388,8 @@ static bool launch_level_lds_w(const DevCaps &caps, const Tune &tune, const DevP
         attr_set = true;
     }
     const int64_t nst = lay.nst[L - 1];
-    const LdsMap map = make_lds_map(caps, tune.occ, nst, nb, lds, WPB, TW, NARROW ? 4 : 1);
+    const LdsMap map = make_lds_map(caps, tune.occ, nst, nb, lds, WPB, TW,
+                                    KIND == KIND_VEC ? 8 : KIND == KIND_NARROW ? 4 : 1);
     LevelArgs<DT, W> a;
     a.trans_lvl = (const W *)pl.d_trans + lay.tstart[L - 1];
     a.nst = nst;
@@ -368,6 +405,7 @@ static bool launch_level_lds_w(const DevCaps &caps, const Tune &tune, const DevP
     a.ldo = ldo;
     a.nb = nb;
     a.map = map;
+    a.addr32 = child_addr32(lay.nstates, ldh * (int64_t)sizeof(typename E::H));
     kern<<<dim3((unsigned)(8 * map.per_xcd)), dim3(WPB * 64), lds, st>>>(a);
     return true;
 }
@@ -376,10 +414,15 @@ template <int DT, typename W, int TAG = 0>
 static bool launch_level_lds(const DevCaps &caps, const Tune &tune, const DevPlan &pl, int L,
                              const void *P, int64_t ldp, const void *Hc, int64_t ldh, void *Ho,
                              int64_t ldo, int64_t nb, hipStream_t st) {
-    if (tune.narrow) {
+    if (tune.vec && vec_rows_ok<DT>(Hc, ldh)) {
         if (tune.wide)
-            return launch_level_lds_w<DT, W, 16, TAG, true>(caps, tune, pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st);
-        return launch_level_lds_w<DT, W, kLdsWaves, TAG, true>(caps, tune, pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st);
+            return launch_level_lds_w<DT, W, 16, TAG, KIND_VEC>(caps, tune, pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st);
+        return launch_level_lds_w<DT, W, kLdsWaves, TAG, KIND_VEC>(caps, tune, pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st);
+    }
+    if (tune.narrow || tune.vec) {
+        if (tune.wide)
+            return launch_level_lds_w<DT, W, 16, TAG, KIND_NARROW>(caps, tune, pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st);
+        return launch_level_lds_w<DT, W, kLdsWaves, TAG, KIND_NARROW>(caps, tune, pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st);
     }
     if (tune.wide)
         return launch_level_lds_w<DT, W, 16, TAG>(caps, tune, pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st);
@@ -457,11 +500,16 @@ static int forward_levels(DevState &ds, const DevPlan &pl, const void *P, int64_
     const int64_t chunk = tune.chunk > 0 ? std::min(tune.chunk, B) : B;
     const size_t hb = sizeof(typename E::H), pb = sizeof(typename E::P);
     const int64_t rows = std::max<int64_t>(lay.max_level, 1);
-    EGG_TRY(ensure(ds.H, (size_t)2 * rows * chunk * hb));
-    char *buf[2] = {(char *)ds.H.p, (char *)ds.H.p + (size_t)rows * chunk * hb};
+    // level rows: nb apart, or for the 16-byte-load kernel padded to whole 128-byte tile rows
+    // (32 instances covers every element type)
+    const bool pad = tune.bt == 64 && tune.lds && tune.vec;
+    const int64_t ldmax = pad ? (chunk + 31) / 32 * 32 : chunk;
+    EGG_TRY(ensure(ds.H, (size_t)2 * rows * ldmax * hb));
+    char *buf[2] = {(char *)ds.H.p, (char *)ds.H.p + (size_t)rows * ldmax * hb};
     const int top = lay.K / 2 + 1;
     for (int64_t b0 = 0; b0 < B; b0 += chunk) {
         const int64_t nb = std::min(chunk, B - b0);
+        const int64_t ld = pad ? (nb + 31) / 32 * 32 : nb;
         const char *Pc = (const char *)P + (size_t)b0 * pb;
         char *outc = (char *)out + (size_t)b0 * hb;
         k_fill_one<DT><<<dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st>>>(
@@ -470,9 +518,9 @@ static int forward_levels(DevState &ds, const DevPlan &pl, const void *P, int64_
             const void *Hc = buf[(L - 1) & 1];
             void *Ho = L == top ? (void *)outc : (void *)buf[L & 1];
             if (tune.bt == 64 && tune.lds &&
-                launch_level_lds<DT, W>(ds.caps, tune, pl, L, Pc, ldp, Hc, nb, Ho, nb, nb, st))
+                launch_level_lds<DT, W>(ds.caps, tune, pl, L, Pc, ldp, Hc, ld, Ho, ld, nb, st))
                 continue;
-            launch_level_bt<DT, W>(tune.bt, pl, L, Pc, ldp, Hc, nb, Ho, nb, nb, st);
+            launch_level_bt<DT, W>(tune.bt, pl, L, Pc, ldp, Hc, ld, Ho, ld, nb, st);
         }
     }
     EGG_HIP(hipGetLastError());
@@ -507,9 +555,8 @@ static int forward_graph(int dtype, DevState &ds, const DevPlan &pl, const void 
     for (auto it = ds.graphs.begin(); it != ds.graphs.end(); ++it) {
         if (it->gen != g_buf_gen) continue;
         if (it->plan == pl.id && it->dtype == dtype && it->P == P && it->ldp == ldp && it->B == B &&
-            it->out == out && it->bt == tune.bt && it->chunk == tune.chunk && it->lds == tune.lds &&
-            it->occ == tune.occ && it->fused == tune.fused && it->wide == tune.wide &&
-            it->narrow == tune.narrow) {
+            it->out == out && it->bt == tune.bt && it->chunk == tune.chunk &&
+            it->flags == tune_flags(tune) && it->occ == tune.occ) {
             EGG_HIP(hipGraphLaunch(it->exec, st));
             return EGGMAN_OK;
         }
@@ -535,11 +582,8 @@ static int forward_graph(int dtype, DevState &ds, const DevPlan &pl, const void 
     ge.out = out;
     ge.bt = tune.bt;
     ge.chunk = tune.chunk;
-    ge.lds = tune.lds;
+    ge.flags = tune_flags(tune);
     ge.occ = tune.occ;
-    ge.fused = tune.fused;
-    ge.wide = tune.wide;
-    ge.narrow = tune.narrow;
     ge.gen = g_buf_gen;
     const hipError_t ei = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
@@ -660,6 +704,7 @@ static int forward_mixed_levels(DevState &ds, const std::vector<DevPlan *> &pls,
                 a.ldo = nb;
                 a.nb = nb;
                 a.map = make_lds_map(ds.caps, tg[g].occ, a.nst, nb, lds, WPB, TW, NARROW ? 4 : 1);
+                a.addr32 = child_addr32(lay.nstates, nb * (int64_t)sizeof(HT));
                 m.first[m.n + 1] = m.first[m.n] + 8 * a.map.per_xcd;
                 m.n++;
                 lds_max = std::max(lds_max, lds);
@@ -693,9 +738,12 @@ static int forward_mixed_dt(DevState &ds, const std::vector<DevPlan *> &pls,
     for (size_t g = 1; g < pls.size(); g++)
         if ((double)pls[g]->lay.ntrans * counts[g] > (double)pls[big]->lay.ntrans * counts[big]) big = g;
     const Tune t = pick_tune(*pls[big], DT, std::max<int64_t>(counts[big], 1));
+    // the mixed launch has no 16-byte-load form (its level rows are not padded): a vec entry
+    // takes the narrow tiles, the other small-tile kernel
+    const bool narrow = t.narrow || t.vec;
     if (pls[0]->lay.index_bytes == 4)
-        return forward_mixed_w<DT, int32_t>(ds, pls, counts, P, out, st, t.wide, t.narrow);
-    return forward_mixed_w<DT, int64_t>(ds, pls, counts, P, out, st, t.wide, t.narrow);
+        return forward_mixed_w<DT, int32_t>(ds, pls, counts, P, out, st, t.wide, narrow);
+    return forward_mixed_w<DT, int64_t>(ds, pls, counts, P, out, st, t.wide, narrow);
 }
 
 static int forward_mixed_any(int dtype, DevState &ds, const std::vector<DevPlan *> &pls,
@@ -1399,6 +1447,15 @@ int eggman_autotune(uint64_t plan, int dtype, int64_t B, int reps, char *report,
                 t.wide = true;
                 cands.push_back(t);
             }
+            for (int occ : {1, 2, 4}) {
+                Tune t;
+                t.chunk = c;
+                t.occ = occ;
+                t.vec = true;
+                cands.push_back(t);
+                t.wide = true;
+                cands.push_back(t);
+            }
             Tune g;
             g.chunk = c;
             g.lds = false;
@@ -1490,21 +1547,11 @@ int eggman_tune_set(int K, int index_bytes, int dtype, int64_t B, int bt, int64_
                     int flags, int occ) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     if (!elt_hbytes(dtype) || B < 1 || (bt & (bt - 1)) || bt < 1 || bt > 64 || occ < 0 ||
-        (flags & ~(TUNE_LDS | TUNE_GRAPH | TUNE_FUSED | TUNE_WIDE | TUNE_CKPT | TUNE_NARROW)))
+        (flags & ~TUNE_ALL))
         return fail(EGGMAN_EINVAL, "bad tuning entry");
     int dev;
     EGG_TRY(current_device(&dev));
-    Tune t;
-    t.bt = bt;
-    t.chunk = chunk;
-    t.occ = occ;
-    t.lds = (flags & TUNE_LDS) != 0;
-    t.graph = (flags & TUNE_GRAPH) != 0;
-    t.fused = (flags & TUNE_FUSED) != 0;
-    t.wide = (flags & TUNE_WIDE) != 0;
-    t.ckpt = (flags & TUNE_CKPT) != 0;
-    t.narrow = (flags & TUNE_NARROW) != 0;
-    g_tune[TuneKey(dev, K, index_bytes, dtype, batch_bucket(B))] = t;
+    g_tune[TuneKey(dev, K, index_bytes, dtype, batch_bucket(B))] = tune_from_flags(bt, chunk, flags, occ);
     return EGGMAN_OK;
 }
 

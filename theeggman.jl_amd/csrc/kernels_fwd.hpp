@@ -123,7 +123,18 @@ struct LevelArgs {
     typename Elt<DT>::H *Ho;                  // this level, rows ldo apart
     int64_t ldo, nb;                          // instances in the group
     LdsMap map;
+    // child addresses fit 32-bit arithmetic: every packed child number is below 2^24 and
+    // (child number + 1) * ldh * sizeof(H) below 2^32 (checked by the launcher)
+    bool addr32;
 };
+
+// Launch-time check behind LevelArgs::addr32 and the other kernels' 32-bit address paths: a
+// child number c <= nstates of the plan, read from a transition word, addresses its row with
+// one 24-bit multiply, c * row_bytes + lane offset, when that stays below 2^32.
+inline bool child_addr32(int64_t nstates, int64_t row_bytes) {
+    return nstates < (int64_t(1) << 24) && row_bytes < (int64_t(1) << 24) &&
+           (nstates + 2) * row_bytes < (int64_t(1) << 32);
+}
 
 // The body of the LDS kernel for workgroup `j` of a launch that gives this level
 // 8 * map.per_xcd workgroups (j counts from a multiple of 8, so j % 8 is still the XCD).
@@ -205,14 +216,13 @@ __device__ inline void level_lds_body(const LevelArgs<DT, W> &a, int64_t j,
 // times as many child rows, and with 8 tiles per pass each XCD owns one tile and walks every
 // state of it.  The level is bound by L2 misses, which this cuts.  Each lane still sums one
 // (state, instance) in plan order: the results are the same bits as the 64-wide kernel's.
-template <int DT, typename W, int WPB>
-__device__ inline void level_lds_narrow_body(const LevelArgs<DT, W> &a, int64_t j,
-                                             typename Elt<DT>::P *lds) {
+template <int DT, typename W, int WPB, bool A32>
+__device__ inline void level_lds_narrow_body_a(const LevelArgs<DT, W> &a, int64_t j,
+                                               typename Elt<DT>::P *lds) {
     using E = Elt<DT>;
     using PT = typename E::P;
     using HT = typename E::H;
     constexpr int bits = Packing<W>::bits;
-    constexpr W mask = (W)((W(1) << bits) - 1);
     constexpr int TW = 16;
     const LdsMap &map = a.map;
     const int lane = (int)(threadIdx.x & 63);
@@ -250,6 +260,13 @@ __device__ inline void level_lds_narrow_body(const LevelArgs<DT, W> &a, int64_t 
 
     const int64_t step = bstride * WPB * 4;
     constexpr int U = 4;
+    // child row c (1-based packed number) of this lane: hbase + c * hrow + hlane (32-bit form);
+    // pair row q (1-based) of this lane in LDS: lbase + q * TW * sizeof(P)
+    const char *hbase = reinterpret_cast<const char *>(a.Hc) - a.cfirst * a.ldh * (int64_t)sizeof(HT);
+    const uint32_t hrow = (uint32_t)(a.ldh * (int64_t)sizeof(HT));
+    const uint32_t hlane = (uint32_t)(bl * (int64_t)sizeof(HT));
+    const char *lbase = reinterpret_cast<const char *>(lds) +
+                        ((int64_t)li - (int64_t)(1 + a.qlo) * TW) * (int64_t)sizeof(PT);
     // a state's words are read once, lane li holding words li and 16 + li, and handed to the
     // group's lanes by a shuffle within the 16-lane group (k <= 31 for the Int32 layout; the
     // Int64 layout, k <= 63, reads four words per lane)
@@ -268,6 +285,18 @@ __device__ inline void level_lds_narrow_body(const LevelArgs<DT, W> &a, int64_t 
                 if ((t >> 4) == c) w = (W)__shfl(wv[c], t & 15, 16);
             return w;
         };
+        auto load_child = [&](W t) -> HT {
+            if constexpr (A32) {
+                const uint32_t off = __umul24((uint32_t)((uint64_t)t >> bits), hrow) + hlane;
+                return *reinterpret_cast<const HT *>(hbase + (uint64_t)off);
+            } else {
+                return a.Hc[((int64_t)(t >> bits) - a.cfirst) * a.ldh + bl];
+            }
+        };
+        auto pair_val = [&](W t) -> PT {
+            const uint32_t q = __builtin_amdgcn_ubfe((uint32_t)t, 0, bits);
+            return *reinterpret_cast<const PT *>(lbase + q * (uint32_t)(TW * sizeof(PT)));
+        };
         HT acc = E::zero();
         int i = 0;
         for (; i + U <= a.k; i += U) {
@@ -276,18 +305,170 @@ __device__ inline void level_lds_narrow_body(const LevelArgs<DT, W> &a, int64_t 
 #pragma unroll
             for (int u = 0; u < U; u++) t[u] = word(i + u);
 #pragma unroll
-            for (int u = 0; u < U; u++) h[u] = a.Hc[((int64_t)(t[u] >> bits) - a.cfirst) * a.ldh + bl];
+            for (int u = 0; u < U; u++) h[u] = load_child(t[u]);
 #pragma unroll
-            for (int u = 0; u < U; u++)
-                acc = E::muladd(acc, lds[((int)(t[u] & mask) - 1 - a.qlo) * TW + li], h[u]);
+            for (int u = 0; u < U; u++) acc = E::muladd(acc, pair_val(t[u]), h[u]);
         }
         for (; i < a.k; i++) {
             const W t = word(i);
-            acc = E::muladd(acc, lds[((int)(t & mask) - 1 - a.qlo) * TW + li],
-                            a.Hc[((int64_t)(t >> bits) - a.cfirst) * a.ldh + bl]);
+            acc = E::muladd(acc, pair_val(t), load_child(t));
         }
         if (ok && live) a.Ho[s * a.ldo + b] = acc;
     }
+}
+
+// The narrow body with 32-bit child addresses when the launch allows them.  The 64-bit form
+// spends about nine integer instructions per transition on the address (three of them
+// quarter-rate multiplies), which bounds the kernel by instruction issue; the 32-bit form is a
+// shift and a 24-bit multiply-add from a uniform base.
+template <int DT, typename W, int WPB>
+__device__ inline void level_lds_narrow_body(const LevelArgs<DT, W> &a, int64_t j,
+                                             typename Elt<DT>::P *lds) {
+    if (a.addr32) level_lds_narrow_body_a<DT, W, WPB, true>(a, j, lds);
+    else level_lds_narrow_body_a<DT, W, WPB, false>(a, j, lds);
+}
+
+// ---------------------------------------------------------------------------------------
+// Vector variant of the narrow kernel: every load is 16 bytes.  A lane holds VEC = 16 / sizeof(H)
+// consecutive instances (two for FP64), 8 lanes cover a state's 128-byte row of a tile of
+// TW = 8 * VEC instances, and a wave walks 8 consecutive states.  The L1 and its address unit
+// process a load four lanes per cycle whatever the width, which bounds the 8-byte kernels:
+// 16-byte loads move twice the data per cycle and halve the address work per value.  Each
+// lane keeps VEC accumulators, every one summed in plan order, so the results are the same bits
+// as the other kernels'.
+//
+// The level rows must be 16-byte aligned with TW | ldh, and a row's padding past the batch must
+// be readable: lanes past the batch read it and store nothing.
+template <int DT, typename W, int WPB, bool A32>
+__device__ inline void level_lds_vec_body_a(const LevelArgs<DT, W> &a, int64_t j,
+                                            typename Elt<DT>::P *lds) {
+    using E = Elt<DT>;
+    using PT = typename E::P;
+    using HT = typename E::H;
+    constexpr int bits = Packing<W>::bits;
+    constexpr int VEC = 16 / (int)sizeof(HT);
+    constexpr int LPS = 8;                  // lanes per state
+    constexpr int TW = LPS * VEC;           // instances per tile
+    constexpr int SPW = 64 / LPS;           // states per wave
+    struct alignas(16) HV { HT v[VEC]; };
+    struct alignas(VEC * sizeof(PT)) PV { PT v[VEC]; };
+    const LdsMap &map = a.map;
+    const int lane = (int)(threadIdx.x & 63);
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int slot = lane / LPS, li = lane % LPS;
+    const int64_t xcd = j & 7, r = j >> 3;
+    int64_t tile, s_begin, s_end, bi, bstride;
+    if (map.tiles_xcd > 0) {
+        const int64_t ti = r % map.tiles_xcd;
+        tile = xcd + 8 * ti;
+        bi = r / map.tiles_xcd;
+        bstride = map.per_xcd / map.tiles_xcd;
+        s_begin = 0;
+        s_end = a.nst;
+        if (tile >= map.ntiles) return;
+    } else {
+        tile = xcd % map.ntiles;
+        const int64_t group = xcd / map.ntiles;
+        const int64_t ngroups = (8 - tile + map.ntiles - 1) / map.ntiles;
+        s_begin = group * a.nst / ngroups;
+        s_end = (group + 1) * a.nst / ngroups;
+        bi = r;
+        bstride = map.per_xcd;
+    }
+    const int64_t b = tile * TW + li * VEC;   // this lane's first instance
+
+    // stage pair rows [qlo, qlo + nq) of this tile: row i at lds[i * TW]
+    for (int e = (int)threadIdx.x; e < a.nq * TW; e += WPB * 64) {
+        const int64_t bb = tile * TW + (e % TW);
+        lds[e] = bb < a.nb ? a.P[(int64_t)(a.qlo + e / TW) * a.ldp + bb] : PT{};
+    }
+    __syncthreads();
+
+    // child row c (1-based packed number) of this lane: hbase + c * hrow + hlane (32-bit form);
+    // pair row q (1-based) of this lane in LDS: lbase + q * TW * sizeof(P)
+    const char *hbase = reinterpret_cast<const char *>(a.Hc) - a.cfirst * a.ldh * (int64_t)sizeof(HT);
+    const uint32_t hrow = (uint32_t)(a.ldh * (int64_t)sizeof(HT));
+    const uint32_t hlane = (uint32_t)(b * (int64_t)sizeof(HT));
+    const char *lbase = reinterpret_cast<const char *>(lds) +
+                        ((int64_t)li * VEC - (int64_t)(1 + a.qlo) * TW) * (int64_t)sizeof(PT);
+    auto load_child = [&](W t) -> HV {
+        if constexpr (A32) {
+            const uint32_t off = __umul24((uint32_t)((uint64_t)t >> bits), hrow) + hlane;
+            return *reinterpret_cast<const HV *>(hbase + (uint64_t)off);
+        } else {
+            return *reinterpret_cast<const HV *>(a.Hc + ((int64_t)(t >> bits) - a.cfirst) * a.ldh + b);
+        }
+    };
+    auto pair_val = [&](W t) -> PV {
+        const uint32_t q = __builtin_amdgcn_ubfe((uint32_t)t, 0, bits);
+        return *reinterpret_cast<const PV *>(lbase + q * (uint32_t)(TW * sizeof(PT)));
+    };
+
+    const int64_t step = bstride * WPB * SPW;
+    constexpr int U = 4;
+    // a state's words are read once, lane li holding words li, li + 8, ..., and handed to the
+    // group's lanes by a shuffle within the 8-lane group
+    constexpr int NW = sizeof(W) == 4 ? 4 : 8;   // k <= 31 (Int32 layout) or 63 (Int64)
+    for (int64_t s0 = s_begin + (bi * WPB + wave) * SPW; s0 < s_end; s0 += step) {
+        const int64_t s = s0 + slot;
+        const bool ok = s < s_end;
+        const W *tw = a.trans_lvl + (ok ? s : s_end - 1) * (int64_t)a.k;
+        W wv[NW];
+#pragma unroll
+        for (int c = 0; c < NW; c++)
+            if (c * LPS < a.k) wv[c] = tw[min(c * LPS + li, a.k - 1)];
+        auto word = [&](int t) -> W {   // t uniform
+            W w = 0;
+#pragma unroll
+            for (int c = 0; c < NW; c++)
+                if (t / LPS == c) w = (W)__shfl(wv[c], t % LPS, LPS);
+            return w;
+        };
+        HT acc[VEC];
+#pragma unroll
+        for (int v = 0; v < VEC; v++) acc[v] = E::zero();
+        int i = 0;
+        for (; i + U <= a.k; i += U) {
+            W t[U];
+            HV h[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) t[u] = word(i + u);
+#pragma unroll
+            for (int u = 0; u < U; u++) h[u] = load_child(t[u]);
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const PV p = pair_val(t[u]);
+#pragma unroll
+                for (int v = 0; v < VEC; v++) acc[v] = E::muladd(acc[v], p.v[v], h[u].v[v]);
+            }
+        }
+        for (; i < a.k; i++) {
+            const W t = word(i);
+            const HV h = load_child(t);
+            const PV p = pair_val(t);
+#pragma unroll
+            for (int v = 0; v < VEC; v++) acc[v] = E::muladd(acc[v], p.v[v], h.v[v]);
+        }
+        if (ok) {
+#pragma unroll
+            for (int v = 0; v < VEC; v++)
+                if (b + v < a.nb) a.Ho[s * a.ldo + b + v] = acc[v];
+        }
+    }
+}
+
+template <int DT, typename W, int WPB>
+__device__ inline void level_lds_vec_body(const LevelArgs<DT, W> &a, int64_t j,
+                                          typename Elt<DT>::P *lds) {
+    if (a.addr32) level_lds_vec_body_a<DT, W, WPB, true>(a, j, lds);
+    else level_lds_vec_body_a<DT, W, WPB, false>(a, j, lds);
+}
+
+template <int DT, typename W, int WPB, int TAG>
+__global__ __launch_bounds__(WPB * 64) void k_level_fwd_lds_vec(LevelArgs<DT, W> a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    level_lds_vec_body<DT, W, WPB>(a, (int64_t)blockIdx.x,
+                                   reinterpret_cast<typename Elt<DT>::P *>(lds_raw));
 }
 
 template <int DT, typename W, int WPB, int TAG>
