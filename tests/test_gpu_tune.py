@@ -149,3 +149,9 @@ def test_vector_loads_give_the_same_bits(K, B, T, chunk):
             code = {np.float64: o.F64, np.float32: o.F32, np.complex128: o.C64, np.complex64: o.C32}[T]
             want = o.haf_dp(synth.pair_batch(As), K, code, 8)
             assert np.asarray(got).tobytes() == want.tobytes()
+    if T != np.int64 and B <= 300:
+        # the gradient's forward pass on padded level rows: the same bits as without
+        _, g_vec = te._haf_dp_backend_grad(BE, P, plan)
+        te.tune_set(K, T, B, bt=64, chunk=chunk, occ=2, lds=True, index_bytes=ib)
+        _, g_wide = te._haf_dp_backend_grad(BE, P, plan)
+        assert np.asarray(g_vec).tobytes() == np.asarray(g_wide).tobytes()

@@ -361,6 +361,10 @@ static bool vec_rows_ok(const void *Hc, int64_t ldh) {
     return ldh % kind_tile<DT>(KIND_VEC) == 0 && ((uintptr_t)Hc & 15) == 0;
 }
 
+// Level rows of internal state buffers: padded to whole 128-byte tile rows of the 16-byte-load
+// kernel (32 instances covers every element type), so any kernel may walk them.
+inline int64_t pad_rows(int64_t nb) { return (nb + 31) / 32 * 32; }
+
 // Launch one level with an LDS kernel; returns false (launching nothing) when the level's pair
 // rows do not fit in one workgroup's LDS.  TAG only separates the instantiation used by the
 // gradient's forward pass from the plain forward in profiles; the code is the same.
@@ -501,15 +505,14 @@ static int forward_levels(DevState &ds, const DevPlan &pl, const void *P, int64_
     const size_t hb = sizeof(typename E::H), pb = sizeof(typename E::P);
     const int64_t rows = std::max<int64_t>(lay.max_level, 1);
     // level rows: nb apart, or for the 16-byte-load kernel padded to whole 128-byte tile rows
-    // (32 instances covers every element type)
     const bool pad = tune.bt == 64 && tune.lds && tune.vec;
-    const int64_t ldmax = pad ? (chunk + 31) / 32 * 32 : chunk;
+    const int64_t ldmax = pad ? pad_rows(chunk) : chunk;
     EGG_TRY(ensure(ds.H, (size_t)2 * rows * ldmax * hb));
     char *buf[2] = {(char *)ds.H.p, (char *)ds.H.p + (size_t)rows * ldmax * hb};
     const int top = lay.K / 2 + 1;
     for (int64_t b0 = 0; b0 < B; b0 += chunk) {
         const int64_t nb = std::min(chunk, B - b0);
-        const int64_t ld = pad ? (nb + 31) / 32 * 32 : nb;
+        const int64_t ld = pad ? pad_rows(nb) : nb;
         const char *Pc = (const char *)P + (size_t)b0 * pb;
         char *outc = (char *)out + (size_t)b0 * hb;
         k_fill_one<DT><<<dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st>>>(
@@ -611,7 +614,7 @@ static int forward_any(int dtype, DevState &ds, const DevPlan &pl, const void *P
 // single-degree walk does) and keeps two-level state buffers of that size; the groups' passes run
 // in rounds, round r walking pass r of every group that has one.  P is the concatenation of the
 // groups' [pair][count] blocks, out of their results.
-template <int DT, typename W, int WPB, bool NARROW>
+template <int DT, typename W, int WPB, int KIND>
 static int forward_mixed_levels(DevState &ds, const std::vector<DevPlan *> &pls,
                                 const std::vector<int64_t> &counts, const void *P, void *out,
                                 hipStream_t st) {
@@ -632,19 +635,19 @@ static int forward_mixed_levels(DevState &ds, const std::vector<DevPlan *> &pls,
         ooff += (size_t)counts[g];
         tg[g] = pick_tune(*pls[g], DT, std::max<int64_t>(counts[g], 1));
         chunk[g] = std::max<int64_t>(1, std::min<int64_t>(tg[g].chunk > 0 ? tg[g].chunk : counts[g], counts[g]));
-        hoff += (size_t)2 * pls[g]->lay.max_level * chunk[g];
+        hoff += (size_t)2 * pls[g]->lay.max_level * pad_rows(chunk[g]);
         if (counts[g] > 0) rounds = std::max<int64_t>(rounds, (counts[g] + chunk[g] - 1) / chunk[g]);
     }
     EGG_TRY(ensure(ds.H, hoff * sizeof(HT)));
     hoff = 0;
     for (size_t g = 0; g < ng; g++) {
-        const size_t rows = (size_t)pls[g]->lay.max_level * chunk[g];
+        const size_t rows = (size_t)pls[g]->lay.max_level * pad_rows(chunk[g]);
         buf0[g] = (HT *)ds.H.p + hoff;
         buf1[g] = buf0[g] + rows;
         hoff += 2 * rows;
     }
-    auto kern = k_level_fwd_lds_multi<DT, W, WPB, NARROW>;
-    constexpr int TW = NARROW ? 16 : 64;
+    auto kern = k_level_fwd_lds_multi<DT, W, WPB, KIND>;
+    constexpr int TW = kind_tile<DT>(KIND);
     static bool attr_set = false;
     if (!attr_set) {
         attr_set = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -678,6 +681,7 @@ static int forward_mixed_levels(DevState &ds, const std::vector<DevPlan *> &pls,
                 const PlanLayout &lay = pl.lay;
                 const int64_t nb = nbg[g];
                 if (nb == 0 || L > lay.K / 2 + 1) continue;
+                const int64_t ld = pad_rows(nb);
                 const PT *Pc = Pg[g] + b0[g];
                 HT *Hc = (L - 1) & 1 ? buf1[g] : buf0[g];
                 HT *Ho = L == lay.K / 2 + 1 ? outg[g] + b0[g] : (L & 1 ? buf1[g] : buf0[g]);
@@ -686,7 +690,7 @@ static int forward_mixed_levels(DevState &ds, const std::vector<DevPlan *> &pls,
                 const size_t lds = (size_t)nq * TW * sizeof(PT);
                 if (!attr_set || !tg[g].lds || lds > ds.caps.max_lds_block) {
                     launch_level_bt<DT, W>(std::min(tg[g].bt, pow2_at_least(nb)), pl, L, Pc, counts[g],
-                                           Hc, nb, Ho, nb, nb, st);
+                                           Hc, ld, Ho, ld, nb, st);
                     continue;
                 }
                 LevelArgs<DT, W> &a = m.g[m.n];
@@ -699,12 +703,13 @@ static int forward_mixed_levels(DevState &ds, const std::vector<DevPlan *> &pls,
                 a.ldp = counts[g];
                 a.Hc = Hc;
                 a.cfirst = lay.levels[L - 2];
-                a.ldh = nb;
+                a.ldh = ld;
                 a.Ho = Ho;
-                a.ldo = nb;
+                a.ldo = ld;
                 a.nb = nb;
-                a.map = make_lds_map(ds.caps, tg[g].occ, a.nst, nb, lds, WPB, TW, NARROW ? 4 : 1);
-                a.addr32 = child_addr32(lay.nstates, nb * (int64_t)sizeof(HT));
+                a.map = make_lds_map(ds.caps, tg[g].occ, a.nst, nb, lds, WPB, TW,
+                                     KIND == KIND_VEC ? 8 : KIND == KIND_NARROW ? 4 : 1);
+                a.addr32 = child_addr32(lay.nstates, ld * (int64_t)sizeof(HT));
                 m.first[m.n + 1] = m.first[m.n] + 8 * a.map.per_xcd;
                 m.n++;
                 lds_max = std::max(lds_max, lds);
@@ -717,17 +722,22 @@ static int forward_mixed_levels(DevState &ds, const std::vector<DevPlan *> &pls,
     return EGGMAN_OK;
 }
 
-// The workgroup width and the tile width follow the tuned choice of the group with the most
+// The workgroup width and the kernel follow the tuned choice of the group with the most
 // transitions (one kernel serves every group of a launch).
 template <int DT, typename W>
 static int forward_mixed_w(DevState &ds, const std::vector<DevPlan *> &pls,
                            const std::vector<int64_t> &counts, const void *P, void *out,
-                           hipStream_t st, bool wide, bool narrow) {
-    if (narrow)
-        return wide ? forward_mixed_levels<DT, W, 16, true>(ds, pls, counts, P, out, st)
-                    : forward_mixed_levels<DT, W, kLdsWaves, true>(ds, pls, counts, P, out, st);
-    return wide ? forward_mixed_levels<DT, W, 16, false>(ds, pls, counts, P, out, st)
-                : forward_mixed_levels<DT, W, kLdsWaves, false>(ds, pls, counts, P, out, st);
+                           hipStream_t st, bool wide, int kind) {
+    switch (kind) {
+    case KIND_VEC:
+        return wide ? forward_mixed_levels<DT, W, 16, KIND_VEC>(ds, pls, counts, P, out, st)
+                    : forward_mixed_levels<DT, W, kLdsWaves, KIND_VEC>(ds, pls, counts, P, out, st);
+    case KIND_NARROW:
+        return wide ? forward_mixed_levels<DT, W, 16, KIND_NARROW>(ds, pls, counts, P, out, st)
+                    : forward_mixed_levels<DT, W, kLdsWaves, KIND_NARROW>(ds, pls, counts, P, out, st);
+    }
+    return wide ? forward_mixed_levels<DT, W, 16, KIND_WIDE>(ds, pls, counts, P, out, st)
+                : forward_mixed_levels<DT, W, kLdsWaves, KIND_WIDE>(ds, pls, counts, P, out, st);
 }
 
 template <int DT>
@@ -738,12 +748,10 @@ static int forward_mixed_dt(DevState &ds, const std::vector<DevPlan *> &pls,
     for (size_t g = 1; g < pls.size(); g++)
         if ((double)pls[g]->lay.ntrans * counts[g] > (double)pls[big]->lay.ntrans * counts[big]) big = g;
     const Tune t = pick_tune(*pls[big], DT, std::max<int64_t>(counts[big], 1));
-    // the mixed launch has no 16-byte-load form (its level rows are not padded): a vec entry
-    // takes the narrow tiles, the other small-tile kernel
-    const bool narrow = t.narrow || t.vec;
+    const int kind = t.vec ? KIND_VEC : t.narrow ? KIND_NARROW : KIND_WIDE;
     if (pls[0]->lay.index_bytes == 4)
-        return forward_mixed_w<DT, int32_t>(ds, pls, counts, P, out, st, t.wide, narrow);
-    return forward_mixed_w<DT, int64_t>(ds, pls, counts, P, out, st, t.wide, narrow);
+        return forward_mixed_w<DT, int32_t>(ds, pls, counts, P, out, st, t.wide, kind);
+    return forward_mixed_w<DT, int64_t>(ds, pls, counts, P, out, st, t.wide, kind);
 }
 
 static int forward_mixed_any(int dtype, DevState &ds, const std::vector<DevPlan *> &pls,
@@ -893,7 +901,8 @@ static bool use_par_dp(const DevCaps &caps, const DevPlan &pl, int dtype) {
 
 template <int DT, int WPB, int U>
 static void launch_dp_par_wu(const DevCaps &caps, const DevPlan &pl, int L, int rows, size_t lds,
-                             const void *Hc, const void *Gs, void *part, int64_t nb, hipStream_t st) {
+                             const void *Hc, int64_t ldh, const void *Gs, void *part, int64_t nb,
+                             hipStream_t st) {
     using HT = typename Elt<DT>::H;
     const PlanLayout &lay = pl.lay;
     auto kern = k_rev_dp_par<DT, WPB, U>;
@@ -911,7 +920,7 @@ static void launch_dp_par_wu(const DevCaps &caps, const DevPlan &pl, int L, int 
     kern<<<dim3((unsigned)blocks), dim3(WPB * 64), lds, st>>>(
         (const int32_t *)pl.d_trans + lay.tstart[L - 1], lay.size[L - 1] - 1, lay.K,
         (const int32_t *)pl.d_prank, (const ParTaskDev *)pl.d_tasks, t0, nt, (const HT *)Hc,
-        lay.levels[L - 2], nb, (const HT *)Gs, nb, (HT *)part, nb, ntiles, rows,
+        lay.levels[L - 2], ldh, (const HT *)Gs, nb, (HT *)part, nb, ntiles, rows,
         (kParTaskLen + WPB - 1) / WPB);
 }
 
@@ -921,27 +930,28 @@ static void launch_dp_par_wu(const DevCaps &caps, const DevPlan &pl, int L, int 
 // wide).
 template <int DT, int WPB>
 static void launch_dp_par_w(const DevCaps &caps, const DevPlan &pl, int L, int rows, size_t lds,
-                            const void *Hc, const void *Gs, void *part, int64_t nb, hipStream_t st) {
+                            const void *Hc, int64_t ldh, const void *Gs, void *part, int64_t nb,
+                            hipStream_t st) {
     const int k = pl.lay.size[L - 1] - 1;
     if constexpr (sizeof(typename Elt<DT>::H) > 8) {
-        launch_dp_par_wu<DT, WPB, 8>(caps, pl, L, rows, lds, Hc, Gs, part, nb, st);
+        launch_dp_par_wu<DT, WPB, 8>(caps, pl, L, rows, lds, Hc, ldh, Gs, part, nb, st);
     } else {
-        if (k <= 8) launch_dp_par_wu<DT, WPB, 8>(caps, pl, L, rows, lds, Hc, Gs, part, nb, st);
+        if (k <= 8) launch_dp_par_wu<DT, WPB, 8>(caps, pl, L, rows, lds, Hc, ldh, Gs, part, nb, st);
         else if (k <= 12 || (k > 16 && k <= 24))
-            launch_dp_par_wu<DT, WPB, 12>(caps, pl, L, rows, lds, Hc, Gs, part, nb, st);
-        else launch_dp_par_wu<DT, WPB, 16>(caps, pl, L, rows, lds, Hc, Gs, part, nb, st);
+            launch_dp_par_wu<DT, WPB, 12>(caps, pl, L, rows, lds, Hc, ldh, Gs, part, nb, st);
+        else launch_dp_par_wu<DT, WPB, 16>(caps, pl, L, rows, lds, Hc, ldh, Gs, part, nb, st);
     }
 }
 
 template <int DT>
 static void launch_dp_par(const DevCaps &caps, const DevPlan &pl, int L, const void *Hc,
-                          const void *Gs, void *part, int64_t nb, hipStream_t st) {
+                          int64_t ldh, const void *Gs, void *part, int64_t nb, hipStream_t st) {
     int rows;
     size_t lds;
     switch (par_waves(caps, pl.lay, L, DT, &rows, &lds)) {
-    case 8: launch_dp_par_w<DT, 8>(caps, pl, L, rows, lds, Hc, Gs, part, nb, st); break;
-    case 4: launch_dp_par_w<DT, 4>(caps, pl, L, rows, lds, Hc, Gs, part, nb, st); break;
-    case 2: launch_dp_par_w<DT, 2>(caps, pl, L, rows, lds, Hc, Gs, part, nb, st); break;
+    case 8: launch_dp_par_w<DT, 8>(caps, pl, L, rows, lds, Hc, ldh, Gs, part, nb, st); break;
+    case 4: launch_dp_par_w<DT, 4>(caps, pl, L, rows, lds, Hc, ldh, Gs, part, nb, st); break;
+    case 2: launch_dp_par_w<DT, 2>(caps, pl, L, rows, lds, Hc, ldh, Gs, part, nb, st); break;
     }
 }
 
@@ -996,7 +1006,10 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
     const int64_t chunk = tune.chunk > 0 ? std::min(tune.chunk, B) : B;
     const bool par = sizeof(W) == 4 && use_par_dp(ds.caps, pl, DT);
     const GradLevels &gl = tune.ckpt ? pl.gl_ckpt : pl.gl_full;
-    EGG_TRY(ensure(ds.Hg, (size_t)std::max<int64_t>(gl.kept_rows + gl.seg_rows, 1) * chunk * hb));
+    // forward level rows: padded to whole tile rows for the 16-byte-load kernel
+    const bool pad = tune.lds && tune.vec;
+    EGG_TRY(ensure(ds.Hg, (size_t)std::max<int64_t>(gl.kept_rows + gl.seg_rows, 1) *
+                              (pad ? pad_rows(chunk) : chunk) * hb));
     EGG_TRY(ensure(ds.G, (size_t)2 * lay.max_level * chunk * hb));
     EGG_TRY(ensure(ds.part, (size_t)std::max<int64_t>(par ? pl.part_rows + pl.nred : pl.nseg, 1) * chunk * hb));
     HT *H = (HT *)ds.Hg.p;
@@ -1008,15 +1021,16 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
         const char *Pc = (const char *)P + (size_t)b0 * pb;
         HT *outc = (HT *)out + b0;
         HT *dPc = (HT *)dP + b0;
+        const int64_t ldh = pad ? pad_rows(nb) : nb;
         // rows of level L (1 .. top-1) in the kept region or the segment region
         auto level_rows = [&](int L) {
-            return H + (size_t)((gl.kept[L] ? 0 : gl.kept_rows) + gl.off[L]) * nb;
+            return H + (size_t)((gl.kept[L] ? 0 : gl.kept_rows) + gl.off[L]) * ldh;
         };
         auto forward_level = [&](int L) {
             const void *Hc = level_rows(L - 1);
             void *Ho = L == top ? (void *)outc : (void *)level_rows(L);
-            if (!(tune.lds && launch_level_lds<DT, W, 1>(ds.caps, tune, pl, L, Pc, ldp, Hc, nb, Ho, nb, nb, st)))
-                launch_level_bt<DT, W>(64, pl, L, Pc, ldp, Hc, nb, Ho, nb, nb, st);
+            if (!(tune.lds && launch_level_lds<DT, W, 1>(ds.caps, tune, pl, L, Pc, ldp, Hc, ldh, Ho, ldh, nb, st)))
+                launch_level_bt<DT, W>(64, pl, L, Pc, ldp, Hc, ldh, Ho, ldh, nb, st);
         };
         k_fill_one<DT><<<dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st>>>(level_rows(1), nb);
         for (int L = 2; L <= top; L++) forward_level(L);
@@ -1038,7 +1052,7 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
                 for (int l = seg_lo; l <= seg_hi; l++) forward_level(l);
             }
             if (par) {
-                launch_dp_par<DT>(ds.caps, pl, L, level_rows(L - 1), G[L & 1], part, nb, st);
+                launch_dp_par<DT>(ds.caps, pl, L, level_rows(L - 1), ldh, G[L & 1], part, nb, st);
             } else {
                 const int64_t s0 = pl.lvl_seg[L - 1], ns = pl.lvl_seg[L] - s0;
                 if (ns > 0) {
@@ -1046,7 +1060,7 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
                     const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (tasks + 3) / 4));
                     k_rev_dp_seg<DT><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
                         (const uint64_t *)pl.d_pent, (const RevSegDev *)pl.d_segs, s0, ns,
-                        level_rows(L - 1), nb, G[L & 1], nb, part, nb, ntiles);
+                        level_rows(L - 1), ldh, G[L & 1], nb, part, nb, ntiles);
                 }
             }
             if (L > 2)
@@ -1139,8 +1153,9 @@ static int grad_tune(const DevPlan &pl, int dtype, int64_t want, Tune *tune, int
     t.chunk = grad_pass(t, want, *cap);
     // narrow tiles pay off when a pass is at most 8 tiles of 16 (one per XCD); the gradient's
     // passes are larger, and there the 64-wide kernel with 16 waves per workgroup, 2 per CU
-    // (its best shape at N = 24..32) measured faster (N=28: 17.7 vs 18.1 ms)
-    if (t.narrow && t.chunk > 128) {
+    // (its best shape at N = 24..32) measured faster (N=28: 17.7 vs 18.1 ms).  The 16-byte-load
+    // kernel keeps its choice.
+    if (t.narrow && !t.vec && t.chunk > 128) {
         t.narrow = false;
         t.wide = true;
         t.occ = 2;

@@ -497,18 +497,19 @@ struct MultiLevelArgs {
     LevelArgs<DT, W> g[kMaxGroups];
 };
 
-template <int DT, typename W, int WPB, bool NARROW>
+template <int DT, typename W, int WPB, int KIND>
 __global__ __launch_bounds__(WPB * 64) void k_level_fwd_lds_multi(MultiLevelArgs<DT, W> m) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int64_t j = blockIdx.x;
     int g = 0;
     while (g + 1 < m.n && j >= m.first[g + 1]) g++;
-    if constexpr (NARROW)
-        level_lds_narrow_body<DT, W, WPB>(m.g[g], j - m.first[g],
-                                          reinterpret_cast<typename Elt<DT>::P *>(lds_raw));
+    auto *lds = reinterpret_cast<typename Elt<DT>::P *>(lds_raw);
+    if constexpr (KIND == 2)        // 16-byte loads
+        level_lds_vec_body<DT, W, WPB>(m.g[g], j - m.first[g], lds);
+    else if constexpr (KIND == 1)   // 16-instance tiles
+        level_lds_narrow_body<DT, W, WPB>(m.g[g], j - m.first[g], lds);
     else
-        level_lds_body<DT, W, WPB>(m.g[g], j - m.first[g],
-                                   reinterpret_cast<typename Elt<DT>::P *>(lds_raw));
+        level_lds_body<DT, W, WPB>(m.g[g], j - m.first[g], lds);
 }
 
 // ---------------------------------------------------------------------------------------
