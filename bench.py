@@ -263,7 +263,8 @@ def main():
     chunk = (tuned or {}).get("chunk") or B
     launches = (K // 2) * (-(-B // min(chunk, B)))
     tl = tuned or {}
-    head = ("k_level_fwd_lds_narrow" if tl.get("narrow") else "k_level_fwd_lds") \
+    head = ("k_level_fwd_lds_vec" if tl.get("vec") else
+            "k_level_fwd_lds_narrow" if tl.get("narrow") else "k_level_fwd_lds") \
         if tl.get("lds", 1) and not tl.get("fused") else "k_level_fwd"
     achieved = nbytes / (ms / 1e3) / 1e9
     result = {

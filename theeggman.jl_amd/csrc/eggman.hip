@@ -901,8 +901,8 @@ static bool use_par_dp(const DevCaps &caps, const DevPlan &pl, int dtype) {
 
 template <int DT, int WPB, int U>
 static void launch_dp_par_wu(const DevCaps &caps, const DevPlan &pl, int L, int rows, size_t lds,
-                             const void *Hc, int64_t ldh, const void *Gs, void *part, int64_t nb,
-                             hipStream_t st) {
+                             const void *Hc, int64_t ldh, const void *Gs, int64_t ldg, void *part,
+                             int64_t nb, hipStream_t st) {
     using HT = typename Elt<DT>::H;
     const PlanLayout &lay = pl.lay;
     auto kern = k_rev_dp_par<DT, WPB, U>;
@@ -920,7 +920,7 @@ static void launch_dp_par_wu(const DevCaps &caps, const DevPlan &pl, int L, int 
     kern<<<dim3((unsigned)blocks), dim3(WPB * 64), lds, st>>>(
         (const int32_t *)pl.d_trans + lay.tstart[L - 1], lay.size[L - 1] - 1, lay.K,
         (const int32_t *)pl.d_prank, (const ParTaskDev *)pl.d_tasks, t0, nt, (const HT *)Hc,
-        lay.levels[L - 2], ldh, (const HT *)Gs, nb, (HT *)part, nb, ntiles, rows,
+        lay.levels[L - 2], ldh, (const HT *)Gs, ldg, (HT *)part, nb, ntiles, rows,
         (kParTaskLen + WPB - 1) / WPB);
 }
 
@@ -930,35 +930,36 @@ static void launch_dp_par_wu(const DevCaps &caps, const DevPlan &pl, int L, int 
 // wide).
 template <int DT, int WPB>
 static void launch_dp_par_w(const DevCaps &caps, const DevPlan &pl, int L, int rows, size_t lds,
-                            const void *Hc, int64_t ldh, const void *Gs, void *part, int64_t nb,
-                            hipStream_t st) {
+                            const void *Hc, int64_t ldh, const void *Gs, int64_t ldg, void *part,
+                            int64_t nb, hipStream_t st) {
     const int k = pl.lay.size[L - 1] - 1;
     if constexpr (sizeof(typename Elt<DT>::H) > 8) {
-        launch_dp_par_wu<DT, WPB, 8>(caps, pl, L, rows, lds, Hc, ldh, Gs, part, nb, st);
+        launch_dp_par_wu<DT, WPB, 8>(caps, pl, L, rows, lds, Hc, ldh, Gs, ldg, part, nb, st);
     } else {
-        if (k <= 8) launch_dp_par_wu<DT, WPB, 8>(caps, pl, L, rows, lds, Hc, ldh, Gs, part, nb, st);
+        if (k <= 8) launch_dp_par_wu<DT, WPB, 8>(caps, pl, L, rows, lds, Hc, ldh, Gs, ldg, part, nb, st);
         else if (k <= 12 || (k > 16 && k <= 24))
-            launch_dp_par_wu<DT, WPB, 12>(caps, pl, L, rows, lds, Hc, ldh, Gs, part, nb, st);
-        else launch_dp_par_wu<DT, WPB, 16>(caps, pl, L, rows, lds, Hc, ldh, Gs, part, nb, st);
+            launch_dp_par_wu<DT, WPB, 12>(caps, pl, L, rows, lds, Hc, ldh, Gs, ldg, part, nb, st);
+        else launch_dp_par_wu<DT, WPB, 16>(caps, pl, L, rows, lds, Hc, ldh, Gs, ldg, part, nb, st);
     }
 }
 
 template <int DT>
 static void launch_dp_par(const DevCaps &caps, const DevPlan &pl, int L, const void *Hc,
-                          int64_t ldh, const void *Gs, void *part, int64_t nb, hipStream_t st) {
+                          int64_t ldh, const void *Gs, int64_t ldg, void *part, int64_t nb,
+                          hipStream_t st) {
     int rows;
     size_t lds;
     switch (par_waves(caps, pl.lay, L, DT, &rows, &lds)) {
-    case 8: launch_dp_par_w<DT, 8>(caps, pl, L, rows, lds, Hc, ldh, Gs, part, nb, st); break;
-    case 4: launch_dp_par_w<DT, 4>(caps, pl, L, rows, lds, Hc, ldh, Gs, part, nb, st); break;
-    case 2: launch_dp_par_w<DT, 2>(caps, pl, L, rows, lds, Hc, ldh, Gs, part, nb, st); break;
+    case 8: launch_dp_par_w<DT, 8>(caps, pl, L, rows, lds, Hc, ldh, Gs, ldg, part, nb, st); break;
+    case 4: launch_dp_par_w<DT, 4>(caps, pl, L, rows, lds, Hc, ldh, Gs, ldg, part, nb, st); break;
+    case 2: launch_dp_par_w<DT, 2>(caps, pl, L, rows, lds, Hc, ldh, Gs, ldg, part, nb, st); break;
     }
 }
 
 template <int DT, typename W>
 static void launch_rev_g(const DevCaps &caps, const Tune &tune, const DevPlan &pl, int L,
-                         const void *P, int64_t ldp, const void *Gs, void *Gc, int64_t nb,
-                         hipStream_t st) {
+                         const void *P, int64_t ldp, const void *Gs, void *Gc, int64_t ldg,
+                         int64_t nb, hipStream_t st) {
     using E = Elt<DT>;
     const PlanLayout &lay = pl.lay;
     int qlo, nq;
@@ -980,13 +981,13 @@ static void launch_rev_g(const DevCaps &caps, const Tune &tune, const DevPlan &p
             const LdsMap map = make_lds_map(caps, tune.occ, nch, nb, lds);
             kern<<<dim3((unsigned)(8 * map.per_xcd)), dim3(kLdsWaves * 64), lds, st>>>(
                 cent, coff, nch, qlo, nq, (const typename E::P *)P, ldp,
-                (const typename E::H *)Gs, nb, (typename E::H *)Gc, nb, map);
+                (const typename E::H *)Gs, ldg, (typename E::H *)Gc, nb, map);
             return;
         }
     }
     const LdsMap map = make_lds_map(caps, 2, nch, nb, 0);
     k_rev_g<DT, W, kLdsWaves, false><<<dim3((unsigned)(8 * map.per_xcd)), dim3(kLdsWaves * 64), 0, st>>>(
-        cent, coff, nch, qlo, nq, (const typename E::P *)P, ldp, (const typename E::H *)Gs, nb,
+        cent, coff, nch, qlo, nq, (const typename E::P *)P, ldp, (const typename E::H *)Gs, ldg,
         (typename E::H *)Gc, nb, map);
 }
 
@@ -1010,10 +1011,11 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
     const bool pad = tune.lds && tune.vec;
     EGG_TRY(ensure(ds.Hg, (size_t)std::max<int64_t>(gl.kept_rows + gl.seg_rows, 1) *
                               (pad ? pad_rows(chunk) : chunk) * hb));
-    EGG_TRY(ensure(ds.G, (size_t)2 * lay.max_level * chunk * hb));
+    const int64_t ldmax = pad ? pad_rows(chunk) : chunk;
+    EGG_TRY(ensure(ds.G, (size_t)2 * lay.max_level * ldmax * hb));
     EGG_TRY(ensure(ds.part, (size_t)std::max<int64_t>(par ? pl.part_rows + pl.nred : pl.nseg, 1) * chunk * hb));
     HT *H = (HT *)ds.Hg.p;
-    HT *G[2] = {(HT *)ds.G.p, (HT *)ds.G.p + (size_t)lay.max_level * chunk};
+    HT *G[2] = {(HT *)ds.G.p, (HT *)ds.G.p + (size_t)lay.max_level * ldmax};
     HT *part = (HT *)ds.part.p;
     const int top = lay.K / 2 + 1;
     for (int64_t b0 = 0; b0 < B; b0 += chunk) {
@@ -1052,7 +1054,7 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
                 for (int l = seg_lo; l <= seg_hi; l++) forward_level(l);
             }
             if (par) {
-                launch_dp_par<DT>(ds.caps, pl, L, level_rows(L - 1), ldh, G[L & 1], part, nb, st);
+                launch_dp_par<DT>(ds.caps, pl, L, level_rows(L - 1), ldh, G[L & 1], ldh, part, nb, st);
             } else {
                 const int64_t s0 = pl.lvl_seg[L - 1], ns = pl.lvl_seg[L] - s0;
                 if (ns > 0) {
@@ -1060,11 +1062,11 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
                     const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (tasks + 3) / 4));
                     k_rev_dp_seg<DT><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
                         (const uint64_t *)pl.d_pent, (const RevSegDev *)pl.d_segs, s0, ns,
-                        level_rows(L - 1), ldh, G[L & 1], nb, part, nb, ntiles);
+                        level_rows(L - 1), ldh, G[L & 1], ldh, part, nb, ntiles);
                 }
             }
             if (L > 2)
-                launch_rev_g<DT, W>(ds.caps, tune, pl, L, Pc, ldp, G[L & 1], G[(L - 1) & 1], nb, st);
+                launch_rev_g<DT, W>(ds.caps, tune, pl, L, Pc, ldp, G[L & 1], G[(L - 1) & 1], ldh, nb, st);
         }
         const int64_t total = lay.npairs * nb;
         const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (total + 255) / 256));
@@ -1439,11 +1441,15 @@ int eggman_autotune(uint64_t plan, int dtype, int64_t B, int reps, char *report,
     EGG_TRY(chain_stream(*ds, ds->stream));
     std::vector<Tune> cands;
     if (B >= 64) {
+        // passes of B / 1 .. B / 16 instances, and of 64, 128 and 256 (the level kernels' own
+        // sweet spots, whatever the batch)
         std::vector<int64_t> chunks;
         for (int64_t d : {1, 2, 4, 8, 16}) {
             int64_t c = (B / d + 63) / 64 * 64;
             if (c >= 64 && std::find(chunks.begin(), chunks.end(), c) == chunks.end()) chunks.push_back(c);
         }
+        for (int64_t c : {64, 128, 256})
+            if (c <= B && std::find(chunks.begin(), chunks.end(), c) == chunks.end()) chunks.push_back(c);
         for (int64_t c : chunks) {
             for (int occ : {1, 2}) {
                 Tune t;

@@ -30,7 +30,7 @@ struct RevSegDev {
 
 // G of level L-1 from G of level L.  One wave per (child, 64-instance tile); the pair rows of
 // level L are staged in LDS as in the forward LDS kernel (same qlo/nq), and the workgroup-to-XCD
-// mapping is the forward one.
+// mapping is the forward one.  Both G levels have rows ldg apart.
 template <int DT, typename W, int WPB, bool USE_LDS>
 __global__ __launch_bounds__(WPB * 64) void k_rev_g(
     const W *__restrict__ cent, const int64_t *__restrict__ coff, int64_t nch, int qlo, int nq,
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(WPB * 64) void k_rev_g(
                 const PT pv = USE_LDS ? lds[q * 64 + lane] : P[(int64_t)(qlo + q) * ldp + b];
                 acc = E::muladd(acc, pv, Gs[s * ldg + b]);
             }
-            Gc[c * nb + b] = acc;
+            Gc[c * ldg + b] = acc;
         }
     }
 }
