@@ -181,6 +181,7 @@ int eggman_dp_forward_grad_device(uint64_t plan, int dtype, const void *P_dev, i
 
 /*
  * Time the launch choices for (plan, dtype, B) -- lanes per state, LDS staging of the pair rows,
+ * the level kernel (64-instance rows, 16-instance tiles or 16-byte loads), waves per workgroup,
  * workgroups per CU, and the instances per pass over the levels -- on synthetic pair values,
  * `reps` runs each, and keep the fastest for this (device, degree, layout, dtype, B rounded up
  * to a power of two).  Later calls of the forward entry points use it; a batch size with no entry
@@ -204,6 +205,9 @@ int eggman_autotune(uint64_t plan, int dtype, int64_t B, int reps, char *report,
                                full pass in device memory) */
 #define EGGMAN_TUNE_NARROW 32 /* LDS kernel with 16-instance tiles and 4 states per wave: a
                                  child row of a tile is one cache line, so L2 holds 4x more */
+#define EGGMAN_TUNE_VEC 64    /* LDS kernel with 16-byte loads: 16 / sizeof(T) instances per
+                                 lane, 8 lanes per state, 8 states per wave (level rows padded
+                                 to whole 128-byte tile rows) */
 
 /*
  * Set / read a tuning entry directly (e.g. to persist tuned choices across processes).  bt:
