@@ -319,6 +319,30 @@ def main():
         hip.device_synchronize()
         result["copy_calibration_bytes"] = nbytes_copy
 
+    if not args.no_plain:
+        # the same batch through the host-pointer entry point (P and the results in host memory,
+        # staged through pinned buffers; uploads and downloads overlap compute across chunks):
+        # the rate a caller handing over host arrays sees, PCIe transfers included
+        Pt = np.ascontiguousarray(P)                              # [pair][batch], host
+        host_out = np.empty(B, np.float64)
+        h = te._device_plan(be, plan)
+
+        def host_call():
+            _lib.check(_lib.lib().eggman_dp_forward(h, _lib.F64, Pt.ctypes.data, B, Pt.shape[0], 0,
+                                                    host_out.ctypes.data), "forward")
+        log("host-pointer entry")
+        host_call()
+        reps = max(5, args.steps // 2)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            host_call()
+        hms = (time.perf_counter() - t0) / reps * 1e3
+        if host_out.tobytes() != res.tobytes():
+            raise SystemExit("host-pointer entry differs from the device entry: refusing to report")
+        result["host_entry"] = {"ms_per_call": hms, "hafnians_per_s": B / (hms / 1e3),
+                                "transitions_per_s": B * nt / (hms / 1e3), "calls": reps,
+                                "bytes_in_per_call": int(P.nbytes), "bytes_out_per_call": B * 8}
+
     if not args.no_grad:
         # forward + full gradient d haf / d P for the same batch, inputs resident in HBM
         dPd = hip.DeviceBuffer(B * 8 * (K * (K - 1) // 2))

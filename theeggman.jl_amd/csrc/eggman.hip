@@ -8,6 +8,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -93,6 +94,12 @@ struct GraphEntry {
 // Bumped whenever a work buffer is reallocated, which invalidates captured graphs.
 static uint64_t g_buf_gen = 1;
 
+// Grow-only pinned host buffer (hipHostMalloc), the staging memory of the host entry points.
+struct PinBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
 struct DevState {
     DevCaps caps;
     hipStream_t cap = nullptr;       // capture stream for graphs
@@ -101,17 +108,19 @@ struct DevState {
     hipStream_t last = nullptr;      // stream of the most recent enqueued call
     hipEvent_t order = nullptr;      // orders calls on different streams (shared buffers)
     DevBuf H;                        // state values (ping-pong levels)
-    DevBuf P;                        // staged pair values (host entry points)
-    DevBuf out;                      // staged results (host entry points)
     DevBuf Hg;                       // every level of H, for the reverse sweep
     DevBuf G;                        // adjoint values (ping-pong levels)
     DevBuf part;                     // per-segment partial gradients
-    DevBuf dP;                       // staged gradients (host entry points)
     DevBuf A;                        // staged matrix (repeated-pattern batches)
-    DevBuf idx;                      // staged index lists (repeated-pattern batches)
+    // host entry points: a two-slot pipeline of pinned host buffers and device buffers, with an
+    // upload and a download stream beside the compute stream (run_pipe)
+    hipStream_t up = nullptr, down = nullptr;
+    hipEvent_t ev_up[2] = {}, ev_comp[2] = {}, ev_down[2] = {};
+    PinBuf pin_in[2], pin_out[2];
+    DevBuf din[2], dout[2];
     size_t bytes() const {
-        return H.bytes + P.bytes + out.bytes + Hg.bytes + G.bytes + part.bytes + dP.bytes +
-               A.bytes + idx.bytes;
+        return H.bytes + Hg.bytes + G.bytes + part.bytes + A.bytes + din[0].bytes + din[1].bytes +
+               dout[0].bytes + dout[1].bytes;
     }
 };
 
@@ -183,6 +192,168 @@ static int find_plan(uint64_t handle, DevPlan **pl) {
         return fail(EGGMAN_EINVAL, "unknown plan handle %llu (released by eggman_cache_clear?)",
                     (unsigned long long)handle);
     *pl = it->second.get();
+    return EGGMAN_OK;
+}
+
+static int ensure_pinned(PinBuf &b, size_t bytes) {
+    if (b.bytes >= bytes) return EGGMAN_OK;
+    if (b.p) {
+        EGG_HIP(hipHostFree(b.p));
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    hipError_t e = hipHostMalloc(&b.p, std::max<size_t>(bytes, 64), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        (void)hipGetLastError();
+        return fail(EGGMAN_ENOMEM, "pinned host allocation of %.3f GB failed: %s", bytes / 1e9,
+                    hipGetErrorString(e));
+    }
+    b.bytes = bytes;
+    return EGGMAN_OK;
+}
+
+// Host copies of the staging.  One thread moves about 5-10 GB/s, below the host link's rate, so
+// copies above a few MB are split over up to 8 threads (by rows, or by bytes for one block).
+template <class F>
+static void split_rows(int64_t rows, size_t bytes, F &&fn) {
+    constexpr size_t kPerThread = size_t(4) << 20;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int64_t nt = std::min<int64_t>(std::min<int64_t>(std::min<int64_t>(8, hw), rows),
+                                         (int64_t)(bytes / kPerThread));
+    if (nt <= 1) {
+        fn((int64_t)0, rows);
+        return;
+    }
+    std::vector<std::thread> th;
+    const int64_t per = (rows + nt - 1) / nt;
+    for (int64_t r0 = per; r0 < rows; r0 += per)
+        th.emplace_back([&fn, r0, per, rows] { fn(r0, std::min(rows, r0 + per)); });
+    fn((int64_t)0, std::min(rows, per));
+    for (auto &t : th) t.join();
+}
+
+static void copy_bytes(void *dst, const void *src, size_t n) {
+    constexpr size_t kBlock = size_t(1) << 20;
+    const int64_t blocks = (int64_t)((n + kBlock - 1) / kBlock);
+    split_rows(blocks, n, [&](int64_t b0, int64_t b1) {
+        const size_t o = (size_t)b0 * kBlock, e = std::min(n, (size_t)b1 * kBlock);
+        std::memcpy((char *)dst + o, (const char *)src + o, e - o);
+    });
+}
+
+// Columns [b0, b0 + nb) of a rows x ld matrix of esz-byte elements (row-major, rows ld apart)
+// into a dense rows x nb block, and back.
+static void copy_columns(void *dst, const void *src, int64_t rows, int64_t ld, int64_t b0,
+                         int64_t nb, size_t esz) {
+    if (nb == ld) {
+        copy_bytes(dst, src, (size_t)rows * nb * esz);
+        return;
+    }
+    split_rows(rows, (size_t)rows * nb * esz, [&](int64_t q0, int64_t q1) {
+        for (int64_t q = q0; q < q1; q++)
+            std::memcpy((char *)dst + (size_t)q * nb * esz,
+                        (const char *)src + ((size_t)q * ld + b0) * esz, (size_t)nb * esz);
+    });
+}
+
+static void scatter_columns(void *dst, const void *src, int64_t rows, int64_t ld, int64_t b0,
+                            int64_t nb, size_t esz) {
+    if (nb == ld) {
+        copy_bytes(dst, src, (size_t)rows * nb * esz);
+        return;
+    }
+    split_rows(rows, (size_t)rows * nb * esz, [&](int64_t q0, int64_t q1) {
+        for (int64_t q = q0; q < q1; q++)
+            std::memcpy((char *)dst + ((size_t)q * ld + b0) * esz,
+                        (const char *)src + (size_t)q * nb * esz, (size_t)nb * esz);
+    });
+}
+
+// One host-entry call as a pipeline over chunks c = 0 .. nchunks-1 through two slots.  For chunk c
+// in slot s = c % 2: the host fills the pinned input (fill(c, pinned, &in_bytes, &out_bytes)), the
+// upload stream copies it to the device, the compute stream runs the chunk (run(c, din, dout,
+// stream)), the download stream copies out_bytes of results back to pinned memory, and the host
+// hands them to the caller (drain(c, pinned)).  Chunk c+1's fill and upload overlap chunk c's
+// compute, and chunk c's download overlaps chunk c+1's compute.  Before slot s is refilled the
+// host waits for its download, which follows its compute, which follows its upload, so no slot is
+// reused while a copy or kernel still reads it.  Every copy has ended when the call returns.
+template <class Fill, class Run, class Drain>
+static int run_pipe(DevState &ds, int64_t nchunks, size_t in_max, size_t out_max, Fill &&fill,
+                    Run &&run, Drain &&drain) {
+    if (!ds.up) {
+        EGG_HIP(hipStreamCreateWithFlags(&ds.up, hipStreamNonBlocking));
+        EGG_HIP(hipStreamCreateWithFlags(&ds.down, hipStreamNonBlocking));
+        for (int s = 0; s < 2; s++) {
+            EGG_HIP(hipEventCreateWithFlags(&ds.ev_up[s], hipEventDisableTiming));
+            EGG_HIP(hipEventCreateWithFlags(&ds.ev_comp[s], hipEventDisableTiming));
+            EGG_HIP(hipEventCreateWithFlags(&ds.ev_down[s], hipEventDisableTiming));
+        }
+    }
+    for (int s = 0; s < 2 && s < nchunks; s++) {
+        EGG_TRY(ensure_pinned(ds.pin_in[s], in_max));
+        EGG_TRY(ensure_pinned(ds.pin_out[s], out_max));
+        EGG_TRY(ensure(ds.din[s], in_max));
+        EGG_TRY(ensure(ds.dout[s], out_max));
+    }
+    hipStream_t st = ds.stream;
+    EGG_TRY(chain_stream(ds, st));
+    auto drained = [&](int rc) {
+        // on an error, let every enqueued copy finish before the caller's memory is released
+        (void)hipStreamSynchronize(ds.up);
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamSynchronize(ds.down);
+        return rc;
+    };
+    for (int64_t c = 0; c < nchunks; c++) {
+        const int s = (int)(c & 1);
+        int rc = EGGMAN_OK;
+        if (c >= 2) {
+            if (hipEventSynchronize(ds.ev_down[s]) != hipSuccess)
+                return drained(fail(EGGMAN_EHIP, "download of chunk %lld failed", (long long)(c - 2)));
+            if ((rc = drain(c - 2, ds.pin_out[s].p)) != EGGMAN_OK) return drained(rc);
+        }
+        size_t ib = 0, ob = 0;
+        if ((rc = fill(c, ds.pin_in[s].p, &ib, &ob)) != EGGMAN_OK) return drained(rc);
+        hipError_t e = hipSuccess;
+        if (ib) e = hipMemcpyAsync(ds.din[s].p, ds.pin_in[s].p, ib, hipMemcpyHostToDevice, ds.up);
+        if (e == hipSuccess) e = hipEventRecord(ds.ev_up[s], ds.up);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, ds.ev_up[s], 0);
+        if (e != hipSuccess) return drained(fail(EGGMAN_EHIP, "upload: %s", hipGetErrorString(e)));
+        if ((rc = run(c, ds.din[s].p, ds.dout[s].p, st)) != EGGMAN_OK) return drained(rc);
+        e = hipEventRecord(ds.ev_comp[s], st);
+        if (e == hipSuccess) e = hipStreamWaitEvent(ds.down, ds.ev_comp[s], 0);
+        if (e == hipSuccess && ob)
+            e = hipMemcpyAsync(ds.pin_out[s].p, ds.dout[s].p, ob, hipMemcpyDeviceToHost, ds.down);
+        if (e == hipSuccess) e = hipEventRecord(ds.ev_down[s], ds.down);
+        if (e != hipSuccess) return drained(fail(EGGMAN_EHIP, "download: %s", hipGetErrorString(e)));
+    }
+    for (int64_t c = std::max<int64_t>(0, nchunks - 2); c < nchunks; c++) {
+        const int s = (int)(c & 1);
+        if (hipEventSynchronize(ds.ev_down[s]) != hipSuccess)
+            return drained(fail(EGGMAN_EHIP, "download of chunk %lld failed", (long long)c));
+        int rc = drain(c, ds.pin_out[s].p);
+        if (rc != EGGMAN_OK) return drained(rc);
+    }
+    return EGGMAN_OK;
+}
+
+// Instances per chunk of a host-entry call: max_batch when the caller gives one; otherwise the
+// whole batch when it is under two passes of the tuned launch, else a quarter of it rounded up
+// to whole passes, so that host staging and transfers overlap compute -- and never more than
+// two slots of per_inst device bytes fit in half of the free device memory.
+static int host_chunk(int64_t B, int64_t max_batch, int64_t pass, size_t per_inst, int64_t *chunk) {
+    if (max_batch > 0) {
+        *chunk = std::min(max_batch, B);
+        return EGGMAN_OK;
+    }
+    size_t free_b = 0, total_b = 0;
+    EGG_HIP(hipMemGetInfo(&free_b, &total_b));
+    const int64_t fit = (int64_t)(free_b / 2 / (2 * std::max<size_t>(per_inst, 1)));
+    pass = std::max<int64_t>(pass, 1);
+    int64_t want = B;
+    if (B >= 2 * pass) want = std::max(pass, ((B + 3) / 4 + pass - 1) / pass * pass);
+    *chunk = std::max<int64_t>(1, std::min(std::min(want, B), fit));
     return EGGMAN_OK;
 }
 
@@ -1376,31 +1547,37 @@ int eggman_dp_forward(uint64_t plan, int dtype, const void *P_host, int64_t B, i
     EGG_TRY(g.rc);
     DevState *ds;
     EGG_TRY(dev_state(pl->device, &ds));
-    EGG_TRY(chain_stream(*ds, ds->stream));
-    const int64_t chunk = max_batch > 0 ? std::min(max_batch, B) : B;
-    EGG_TRY(ensure(ds->P, (size_t)chunk * npairs * pb));
-    EGG_TRY(ensure(ds->out, (size_t)chunk * hb));
+    const Tune t0 = pick_tune(*pl, dtype, B);
+    int64_t chunk;
+    EGG_TRY(host_chunk(B, max_batch, t0.chunk > 0 ? t0.chunk : B, (size_t)npairs * pb + hb, &chunk));
     const Tune tune = pick_tune(*pl, dtype, chunk);
-    std::vector<char> stage;
-    for (int64_t b0 = 0; b0 < B; b0 += chunk) {
-        const int64_t nb = std::min(chunk, B - b0);
-        const void *src = P_host;
-        if (nb != B) {
+    const int64_t nchunks = (B + chunk - 1) / chunk;
+    auto span = [&](int64_t c, int64_t *b0, int64_t *nb) {
+        *b0 = c * chunk;
+        *nb = std::min(chunk, B - *b0);
+    };
+    return run_pipe(
+        *ds, nchunks, (size_t)chunk * npairs * pb, (size_t)chunk * hb,
+        [&](int64_t c, void *pin, size_t *ib, size_t *ob) {
+            int64_t b0, nb;
+            span(c, &b0, &nb);
             // this chunk's pairs, relaid out to [pair][nb] (ext/TheEggmanKernelAbstractionsExt.jl:127-129)
-            stage.resize((size_t)nb * npairs * pb);
-            for (int64_t q = 0; q < npairs; q++)
-                std::memcpy(stage.data() + (size_t)q * nb * pb,
-                            (const char *)P_host + ((size_t)q * B + b0) * pb, (size_t)nb * pb);
-            src = stage.data();
-        }
-        EGG_HIP(hipMemcpyAsync(ds->P.p, src, (size_t)nb * npairs * pb, hipMemcpyHostToDevice,
-                               ds->stream));
-        EGG_TRY(forward_any(dtype, *ds, *pl, ds->P.p, nb, nb, ds->out.p, ds->stream, tune));
-        EGG_HIP(hipMemcpyAsync((char *)out_host + (size_t)b0 * hb, ds->out.p, (size_t)nb * hb,
-                               hipMemcpyDeviceToHost, ds->stream));
-        EGG_HIP(hipStreamSynchronize(ds->stream));
-    }
-    return EGGMAN_OK;
+            copy_columns(pin, P_host, npairs, B, b0, nb, pb);
+            *ib = (size_t)nb * npairs * pb;
+            *ob = (size_t)nb * hb;
+            return EGGMAN_OK;
+        },
+        [&](int64_t c, void *din, void *dout, hipStream_t st) {
+            int64_t b0, nb;
+            span(c, &b0, &nb);
+            return forward_any(dtype, *ds, *pl, din, nb, nb, dout, st, tune);
+        },
+        [&](int64_t c, const void *pin) {
+            int64_t b0, nb;
+            span(c, &b0, &nb);
+            std::memcpy((char *)out_host + (size_t)b0 * hb, pin, (size_t)nb * hb);
+            return EGGMAN_OK;
+        });
 }
 
 int eggman_dp_forward_device(uint64_t plan, int dtype, const void *P_dev, int64_t ldp, int64_t B,
@@ -1611,25 +1788,45 @@ int eggman_dp_forward_repeated(uint64_t plan, int dtype, const void *A_host, int
     DevState *ds;
     EGG_TRY(dev_state(pl->device, &ds));
     EGG_TRY(chain_stream(*ds, ds->stream));
-    const int64_t chunk = max_batch > 0 ? std::min(max_batch, B) : B;
     const int64_t npairs = pl->lay.npairs;
-    EGG_TRY(ensure(ds->P, (size_t)chunk * npairs * pb));
-    EGG_TRY(ensure(ds->out, (size_t)chunk * hb));
-    EGG_TRY(ensure(ds->A, (size_t)N * N * pb));
-    EGG_TRY(ensure(ds->idx, (size_t)chunk * K * sizeof(int32_t)));
-    EGG_HIP(hipMemcpyAsync(ds->A.p, A_host, (size_t)N * N * pb, hipMemcpyHostToDevice, ds->stream));
+    const Tune t0 = pick_tune(*pl, dtype, B);
+    int64_t chunk;
+    EGG_TRY(host_chunk(B, max_batch, t0.chunk > 0 ? t0.chunk : B,
+                       (size_t)K * sizeof(int32_t) + hb + (size_t)npairs * pb / 2, &chunk));
     const Tune tune = pick_tune(*pl, dtype, chunk);
-    for (int64_t b0 = 0; b0 < B; b0 += chunk) {
-        const int64_t nb = std::min(chunk, B - b0);
-        EGG_HIP(hipMemcpyAsync(ds->idx.p, idx_host + b0 * K, (size_t)nb * K * sizeof(int32_t),
-                               hipMemcpyHostToDevice, ds->stream));
-        gather_any(dtype, ds->A.p, N, (const int32_t *)ds->idx.p, K, nb, ds->P.p, nb, ds->stream);
-        EGG_TRY(forward_any(dtype, *ds, *pl, ds->P.p, nb, nb, ds->out.p, ds->stream, tune));
-        EGG_HIP(hipMemcpyAsync((char *)out_host + (size_t)b0 * hb, ds->out.p, (size_t)nb * hb,
-                               hipMemcpyDeviceToHost, ds->stream));
-        EGG_HIP(hipStreamSynchronize(ds->stream));
-    }
-    return EGGMAN_OK;
+    const int64_t nchunks = (B + chunk - 1) / chunk;
+    // the matrix once, then the pair values of each chunk gathered on the device into one buffer
+    // (chunks run one after another on the compute stream)
+    const size_t aoff = ((size_t)N * N * pb + 255) & ~(size_t)255;
+    EGG_TRY(ensure(ds->A, aoff + (size_t)chunk * npairs * pb));
+    char *Pg = (char *)ds->A.p + aoff;
+    EGG_HIP(hipMemcpyAsync(ds->A.p, A_host, (size_t)N * N * pb, hipMemcpyHostToDevice, ds->stream));
+    auto span = [&](int64_t c, int64_t *b0, int64_t *nb) {
+        *b0 = c * chunk;
+        *nb = std::min(chunk, B - *b0);
+    };
+    return run_pipe(
+        *ds, nchunks, (size_t)chunk * K * sizeof(int32_t), (size_t)chunk * hb,
+        [&](int64_t c, void *pin, size_t *ib, size_t *ob) {
+            int64_t b0, nb;
+            span(c, &b0, &nb);
+            std::memcpy(pin, idx_host + b0 * K, (size_t)nb * K * sizeof(int32_t));
+            *ib = (size_t)nb * K * sizeof(int32_t);
+            *ob = (size_t)nb * hb;
+            return EGGMAN_OK;
+        },
+        [&](int64_t c, void *din, void *dout, hipStream_t st) {
+            int64_t b0, nb;
+            span(c, &b0, &nb);
+            gather_any(dtype, ds->A.p, N, (const int32_t *)din, K, nb, Pg, nb, st);
+            return forward_any(dtype, *ds, *pl, Pg, nb, nb, dout, st, tune);
+        },
+        [&](int64_t c, const void *pin) {
+            int64_t b0, nb;
+            span(c, &b0, &nb);
+            std::memcpy((char *)out_host + (size_t)b0 * hb, pin, (size_t)nb * hb);
+            return EGGMAN_OK;
+        });
 }
 
 int eggman_dp_forward_mixed(int ngroups, const uint64_t *plans, const int64_t *counts, int dtype,
@@ -1647,14 +1844,22 @@ int eggman_dp_forward_mixed(int ngroups, const uint64_t *plans, const int64_t *c
     EGG_TRY(g.rc);
     DevState *ds;
     EGG_TRY(dev_state(pls[0]->device, &ds));
-    EGG_TRY(chain_stream(*ds, ds->stream));
-    EGG_TRY(ensure(ds->P, (size_t)np_total * pb));
-    EGG_TRY(ensure(ds->out, (size_t)total * hb));
-    EGG_HIP(hipMemcpyAsync(ds->P.p, P_host, (size_t)np_total * pb, hipMemcpyHostToDevice, ds->stream));
-    EGG_TRY(forward_mixed_any(dtype, *ds, pls, cnt, ds->P.p, ds->out.p, ds->stream));
-    EGG_HIP(hipMemcpyAsync(out_host, ds->out.p, (size_t)total * hb, hipMemcpyDeviceToHost, ds->stream));
-    EGG_HIP(hipStreamSynchronize(ds->stream));
-    return EGGMAN_OK;
+    // one chunk: every group advances together in one device pass
+    return run_pipe(
+        *ds, 1, (size_t)np_total * pb, (size_t)total * hb,
+        [&](int64_t, void *pin, size_t *ib, size_t *ob) {
+            copy_bytes(pin, P_host, (size_t)np_total * pb);
+            *ib = (size_t)np_total * pb;
+            *ob = (size_t)total * hb;
+            return EGGMAN_OK;
+        },
+        [&](int64_t, void *din, void *dout, hipStream_t st) {
+            return forward_mixed_any(dtype, *ds, pls, cnt, din, dout, st);
+        },
+        [&](int64_t, const void *pin) {
+            std::memcpy(out_host, pin, (size_t)total * hb);
+            return EGGMAN_OK;
+        });
 }
 
 int eggman_dp_forward_mixed_device(int ngroups, const uint64_t *plans, const int64_t *counts,
@@ -1693,46 +1898,41 @@ int eggman_dp_forward_grad(uint64_t plan, int dtype, const void *P_host, int64_t
     EGG_TRY(g.rc);
     DevState *ds;
     EGG_TRY(dev_state(pl->device, &ds));
-    EGG_TRY(chain_stream(*ds, ds->stream));
     EGG_TRY(ensure_rev(*pl));
     Tune tune;
-    int64_t chunk;
-    EGG_TRY(grad_tune(*pl, dtype, max_batch > 0 ? std::min(max_batch, B) : B, &tune, &chunk));
-    EGG_TRY(ensure(ds->P, (size_t)chunk * npairs * pb));
-    EGG_TRY(ensure(ds->dP, (size_t)chunk * npairs * hb));
-    EGG_TRY(ensure(ds->out, (size_t)chunk * hb));
-    std::vector<char> stage, gstage;
-    for (int64_t b0 = 0; b0 < B; b0 += chunk) {
-        const int64_t nb = std::min(chunk, B - b0);
-        const void *src = P_host;
-        if (nb != B) {
-            stage.resize((size_t)nb * npairs * pb);
-            for (int64_t q = 0; q < npairs; q++)
-                std::memcpy(stage.data() + (size_t)q * nb * pb,
-                            (const char *)P_host + ((size_t)q * B + b0) * pb, (size_t)nb * pb);
-            src = stage.data();
-        }
-        EGG_HIP(hipMemcpyAsync(ds->P.p, src, (size_t)nb * npairs * pb, hipMemcpyHostToDevice,
-                               ds->stream));
-        EGG_TRY(forward_grad_any(dtype, *ds, *pl, ds->P.p, nb, nb, ds->out.p, ds->dP.p, nb,
-                                 ds->stream, tune));
-        EGG_HIP(hipMemcpyAsync((char *)out_host + (size_t)b0 * hb, ds->out.p, (size_t)nb * hb,
-                               hipMemcpyDeviceToHost, ds->stream));
-        if (nb == B) {
-            EGG_HIP(hipMemcpyAsync(dP_host, ds->dP.p, (size_t)nb * npairs * hb,
-                                   hipMemcpyDeviceToHost, ds->stream));
-            EGG_HIP(hipStreamSynchronize(ds->stream));
-        } else {
-            gstage.resize((size_t)nb * npairs * hb);
-            EGG_HIP(hipMemcpyAsync(gstage.data(), ds->dP.p, gstage.size(), hipMemcpyDeviceToHost,
-                                   ds->stream));
-            EGG_HIP(hipStreamSynchronize(ds->stream));
-            for (int64_t q = 0; q < npairs; q++)
-                std::memcpy((char *)dP_host + ((size_t)q * B + b0) * hb,
-                            gstage.data() + (size_t)q * nb * hb, (size_t)nb * hb);
-        }
-    }
-    return EGGMAN_OK;
+    int64_t cap, chunk;
+    EGG_TRY(grad_tune(*pl, dtype, max_batch > 0 ? std::min(max_batch, B) : B, &tune, &cap));
+    EGG_TRY(host_chunk(B, max_batch, tune.chunk, (size_t)npairs * (pb + hb) + hb, &chunk));
+    chunk = std::min(chunk, cap);
+    const int64_t nchunks = (B + chunk - 1) / chunk;
+    auto span = [&](int64_t c, int64_t *b0, int64_t *nb) {
+        *b0 = c * chunk;
+        *nb = std::min(chunk, B - *b0);
+    };
+    // a chunk's results: its nb values, then its gradient rows [pair][nb]
+    return run_pipe(
+        *ds, nchunks, (size_t)chunk * npairs * pb, (size_t)chunk * (npairs + 1) * hb,
+        [&](int64_t c, void *pin, size_t *ib, size_t *ob) {
+            int64_t b0, nb;
+            span(c, &b0, &nb);
+            copy_columns(pin, P_host, npairs, B, b0, nb, pb);
+            *ib = (size_t)nb * npairs * pb;
+            *ob = (size_t)nb * (npairs + 1) * hb;
+            return EGGMAN_OK;
+        },
+        [&](int64_t c, void *din, void *dout, hipStream_t st) {
+            int64_t b0, nb;
+            span(c, &b0, &nb);
+            return forward_grad_any(dtype, *ds, *pl, din, nb, nb, dout,
+                                    (char *)dout + (size_t)nb * hb, nb, st, tune);
+        },
+        [&](int64_t c, const void *pin) {
+            int64_t b0, nb;
+            span(c, &b0, &nb);
+            std::memcpy((char *)out_host + (size_t)b0 * hb, pin, (size_t)nb * hb);
+            scatter_columns(dP_host, (const char *)pin + (size_t)nb * hb, npairs, B, b0, nb, hb);
+            return EGGMAN_OK;
+        });
 }
 
 int eggman_dp_forward_grad_device(uint64_t plan, int dtype, const void *P_dev, int64_t ldp,
@@ -1777,10 +1977,15 @@ int eggman_cache_clear(void) {
         if (hipDeviceSynchronize() != hipSuccess) rc = fail(EGGMAN_EHIP, "synchronize failed");
         for (GraphEntry &ge : kv.second.graphs) (void)hipGraphExecDestroy(ge.exec);
         kv.second.graphs.clear();
-        for (DevBuf *b : {&kv.second.H, &kv.second.P, &kv.second.out, &kv.second.Hg,
-                          &kv.second.G, &kv.second.part, &kv.second.dP, &kv.second.A,
-                          &kv.second.idx}) {
+        DevState &d = kv.second;
+        for (DevBuf *b : {&d.H, &d.Hg, &d.G, &d.part, &d.A, &d.din[0], &d.din[1], &d.dout[0],
+                          &d.dout[1]}) {
             if (b->p) (void)hipFree(b->p);
+            b->p = nullptr;
+            b->bytes = 0;
+        }
+        for (PinBuf *b : {&d.pin_in[0], &d.pin_in[1], &d.pin_out[0], &d.pin_out[1]}) {
+            if (b->p) (void)hipHostFree(b->p);
             b->p = nullptr;
             b->bytes = 0;
         }
