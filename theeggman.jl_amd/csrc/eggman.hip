@@ -108,6 +108,7 @@ struct DevState {
     hipStream_t last = nullptr;      // stream of the most recent enqueued call
     hipEvent_t order = nullptr;      // orders calls on different streams (shared buffers)
     DevBuf H;                        // state values (ping-pong levels)
+    DevBuf Hw;                       // whole-batch rows of the small levels
     DevBuf Hg;                       // every level of H, for the reverse sweep
     DevBuf G;                        // adjoint values (ping-pong levels)
     DevBuf part;                     // per-segment partial gradients
@@ -119,7 +120,7 @@ struct DevState {
     PinBuf pin_in[2], pin_out[2];
     DevBuf din[2], dout[2];
     size_t bytes() const {
-        return H.bytes + Hg.bytes + G.bytes + part.bytes + A.bytes + din[0].bytes + din[1].bytes +
+        return H.bytes + Hw.bytes + Hg.bytes + G.bytes + part.bytes + A.bytes + din[0].bytes + din[1].bytes +
                dout[0].bytes + dout[1].bytes;
     }
 };
@@ -660,6 +661,11 @@ static bool launch_fused(const DevCaps &caps, const DevPlan &pl, const void *P, 
     return true;
 }
 
+// Levels with at most kSmallLevel states are walked once for the whole batch rather than once per
+// pass, while their whole-batch rows fit in kSmallLevelBytes.
+constexpr int64_t kSmallLevel = 4096;
+constexpr size_t kSmallLevelBytes = size_t(1) << 30;
+
 // Walk all levels for instances [0, B) of P (rows ldp apart) into out, `chunk` instances per
 // pass, keeping only two levels of state values (the level being written and the level below,
 // which holds every child it reads).
@@ -678,25 +684,74 @@ static int forward_levels(DevState &ds, const DevPlan &pl, const void *P, int64_
     // level rows: nb apart, or for the 16-byte-load kernel padded to whole 128-byte tile rows
     const bool pad = tune.bt == 64 && tune.lds && tune.vec;
     const int64_t ldmax = pad ? pad_rows(chunk) : chunk;
+    const int top = lay.K / 2 + 1;
+    auto launch = [&](int L, const char *Pc, const void *Hc, int64_t ldh, void *Ho, int64_t ldo,
+                      int64_t nb) {
+        if (tune.bt == 64 && tune.lds &&
+            launch_level_lds<DT, W>(ds.caps, tune, pl, L, Pc, ldp, Hc, ldh, Ho, ldo, nb, st))
+            return;
+        launch_level_bt<DT, W>(tune.bt, pl, L, Pc, ldp, Hc, ldh, Ho, ldo, nb, st);
+    };
+    // Small levels at the bottom (2 .. lb) and the top (lt .. top) are walked once for the whole
+    // batch; the passes walk lb+1 .. lt-1, reading level lb's whole-batch rows at their columns
+    // and writing level lt-1's.  Small levels are bound by latency (few states, long sums), so
+    // one launch over every instance costs about what one pass's launch did.
+    int lb = 1, lt = top + 1;
+    const int64_t ldw = pad ? pad_rows(B) : B;
+    int64_t wrows_b = 0, wrows_t = 0;
+    if (chunk < B && tune.bt == 64) {
+        while (lb + 1 < top && lay.nst[lb] <= kSmallLevel) lb++;
+        while (lt - 1 > lb + 1 && lay.nst[lt - 2] <= kSmallLevel) lt--;
+        for (int L = 1; L <= lb && lb > 1; L++) wrows_b = std::max(wrows_b, lay.nst[L - 1]);
+        for (int L = lt - 1; L < top && lt <= top; L++) wrows_t = std::max(wrows_t, lay.nst[L - 1]);
+        if ((size_t)2 * (wrows_b + wrows_t) * ldw * hb > kSmallLevelBytes) {
+            lb = 1;
+            lt = top + 1;
+            wrows_b = wrows_t = 0;
+        }
+    }
     EGG_TRY(ensure(ds.H, (size_t)2 * rows * ldmax * hb));
     char *buf[2] = {(char *)ds.H.p, (char *)ds.H.p + (size_t)rows * ldmax * hb};
-    const int top = lay.K / 2 + 1;
+    char *wb[2] = {nullptr, nullptr}, *wt[2] = {nullptr, nullptr};
+    if (wrows_b + wrows_t > 0) {
+        EGG_TRY(ensure(ds.Hw, (size_t)2 * (wrows_b + wrows_t) * ldw * hb));
+        char *w = (char *)ds.Hw.p;
+        wb[0] = w;
+        wb[1] = w + (size_t)wrows_b * ldw * hb;
+        wt[0] = w + (size_t)2 * wrows_b * ldw * hb;
+        wt[1] = wt[0] + (size_t)wrows_t * ldw * hb;
+    }
+    if (lb > 1) {
+        k_fill_one<DT><<<dim3((unsigned)((B + 255) / 256)), dim3(256), 0, st>>>(
+            (typename E::H *)wb[1], B);
+        for (int L = 2; L <= lb; L++)
+            launch(L, (const char *)P, wb[(L - 1) & 1], ldw, wb[L & 1], ldw, B);
+    }
     for (int64_t b0 = 0; b0 < B; b0 += chunk) {
         const int64_t nb = std::min(chunk, B - b0);
         const int64_t ld = pad ? pad_rows(nb) : nb;
         const char *Pc = (const char *)P + (size_t)b0 * pb;
         char *outc = (char *)out + (size_t)b0 * hb;
-        k_fill_one<DT><<<dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st>>>(
-            (typename E::H *)buf[1], nb);
-        for (int L = 2; L <= top; L++) {
-            const void *Hc = buf[(L - 1) & 1];
-            void *Ho = L == top ? (void *)outc : (void *)buf[L & 1];
-            if (tune.bt == 64 && tune.lds &&
-                launch_level_lds<DT, W>(ds.caps, tune, pl, L, Pc, ldp, Hc, ld, Ho, ld, nb, st))
-                continue;
-            launch_level_bt<DT, W>(tune.bt, pl, L, Pc, ldp, Hc, ld, Ho, ld, nb, st);
+        if (lb == 1)
+            k_fill_one<DT><<<dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st>>>(
+                (typename E::H *)buf[1], nb);
+        for (int L = lb + 1; L <= lt - 1; L++) {
+            const bool from_w = L - 1 == lb && lb > 1;
+            const void *Hc = from_w ? (const void *)(wb[lb & 1] + (size_t)b0 * hb) : buf[(L - 1) & 1];
+            const int64_t ldh = from_w ? ldw : ld;
+            void *Ho = buf[L & 1];
+            int64_t ldo = ld;
+            if (L == top) {
+                Ho = outc;
+            } else if (L == lt - 1 && lt <= top) {
+                Ho = wt[L & 1] + (size_t)b0 * hb;
+                ldo = ldw;
+            }
+            launch(L, Pc, Hc, ldh, Ho, ldo, nb);
         }
     }
+    for (int L = lt; L <= top; L++)
+        launch(L, (const char *)P, wt[(L - 1) & 1], ldw, L == top ? out : (void *)wt[L & 1], ldw, B);
     EGG_HIP(hipGetLastError());
     return EGGMAN_OK;
 }
@@ -1978,7 +2033,7 @@ int eggman_cache_clear(void) {
         for (GraphEntry &ge : kv.second.graphs) (void)hipGraphExecDestroy(ge.exec);
         kv.second.graphs.clear();
         DevState &d = kv.second;
-        for (DevBuf *b : {&d.H, &d.Hg, &d.G, &d.part, &d.A, &d.din[0], &d.din[1], &d.dout[0],
+        for (DevBuf *b : {&d.H, &d.Hw, &d.Hg, &d.G, &d.part, &d.A, &d.din[0], &d.din[1], &d.dout[0],
                           &d.dout[1]}) {
             if (b->p) (void)hipFree(b->p);
             b->p = nullptr;
