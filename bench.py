@@ -36,6 +36,23 @@ def alg_bytes_fwd(K, B, tbytes, ibytes, ns, nt):
     return (nt + ns + 1) * ibytes + B * (nt + ns + npairs) * tbytes
 
 
+def level_launches(nst, B, chunk, small=4096):
+    """Level launches of one forward call of B instances in passes of `chunk`, as the library
+    makes them: every level once per pass, except that when there is more than one pass the
+    small levels at the bottom and the top (at most `small` states each) run once for the whole
+    batch.  nst[L-1] is the number of states of level L."""
+    top = len(nst)
+    passes = -(-B // chunk)
+    if passes == 1:
+        return top - 1
+    lb, lt = 1, top + 1
+    while lb + 1 < top and nst[lb] <= small:
+        lb += 1
+    while lt - 1 > lb + 1 and nst[lt - 2] <= small:
+        lt -= 1
+    return (lb - 1) + passes * ((lt - 1) - lb) + (top - lt + 1)
+
+
 def time_device(hip, stream, fn, steps, warmup):
     """Run fn warmup times, then time exactly `steps` calls between synchronisations, with HIP
     events recorded on `stream`, the stream the kernels are launched on."""
@@ -259,9 +276,8 @@ def main():
 
     value = B * nt / (ms / 1e3)
     nbytes = alg_bytes_fwd(K, B, 8, ib, ns, nt)
-    # level launches per step: K/2 levels for every pass of `chunk` instances
     chunk = (tuned or {}).get("chunk") or B
-    launches = (K // 2) * (-(-B // min(chunk, B)))
+    launches = level_launches(np.diff(np.asarray(plan.levels, dtype=np.int64)), B, min(chunk, B))
     tl = tuned or {}
     head = ("k_level_fwd_lds_vec" if tl.get("vec") else
             "k_level_fwd_lds_narrow" if tl.get("narrow") else "k_level_fwd_lds") \
