@@ -386,6 +386,7 @@ struct Tune {
     bool ckpt = false;  // gradient: keep checkpoint levels only and recompute the others
     bool narrow = false;// LDS kernel with 16-instance tiles, 4 states per wave
     bool vec = false;   // LDS kernel with 16-byte loads: 8 lanes per state, 8 states per wave
+    int64_t sub = 0;    // gradient: instances per forward sub-pass inside a pass (0 = the pass)
 };
 
 // Tuning flags as they cross the C ABI (eggman_tune_set / eggman_tune_get).
@@ -1254,14 +1255,23 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
         auto level_rows = [&](int L) {
             return H + (size_t)((gl.kept[L] ? 0 : gl.kept_rows) + gl.off[L]) * ldh;
         };
-        auto forward_level = [&](int L) {
-            const void *Hc = level_rows(L - 1);
-            void *Ho = L == top ? (void *)outc : (void *)level_rows(L);
-            if (!(tune.lds && launch_level_lds<DT, W, 1>(ds.caps, tune, pl, L, Pc, ldp, Hc, ldh, Ho, ldh, nb, st)))
-                launch_level_bt<DT, W>(64, pl, L, Pc, ldp, Hc, ldh, Ho, ldh, nb, st);
+        // levels L0 .. L1 of the pass, walked in sub-passes of the forward's own tuned size (its
+        // child rows stay in L2 as in the forward alone), each writing its columns of the rows
+        const int64_t sub = tune.sub > 0 ? std::min(tune.sub, nb) : nb;
+        auto forward_range = [&](int L0, int L1) {
+            for (int64_t s0 = 0; s0 < nb; s0 += sub) {
+                const int64_t ns = std::min(sub, nb - s0);
+                const char *Ps = Pc + (size_t)s0 * pb;
+                for (int L = L0; L <= L1; L++) {
+                    const void *Hc = level_rows(L - 1) + s0;
+                    void *Ho = L == top ? (void *)(outc + s0) : (void *)(level_rows(L) + s0);
+                    if (!(tune.lds && launch_level_lds<DT, W, 1>(ds.caps, tune, pl, L, Ps, ldp, Hc, ldh, Ho, ldh, ns, st)))
+                        launch_level_bt<DT, W>(64, pl, L, Ps, ldp, Hc, ldh, Ho, ldh, ns, st);
+                }
+            }
         };
         k_fill_one<DT><<<dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st>>>(level_rows(1), nb);
-        for (int L = 2; L <= top; L++) forward_level(L);
+        forward_range(2, top);
         // the segment the forward left in the segment region: the run of levels below top
         int seg_lo = top, seg_hi = top - 1;
         if (!gl.kept[top - 1]) {
@@ -1277,7 +1287,7 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
                 seg_hi = L - 1;
                 seg_lo = L - 1;
                 while (!gl.kept[seg_lo - 1]) seg_lo--;
-                for (int l = seg_lo; l <= seg_hi; l++) forward_level(l);
+                forward_range(seg_lo, seg_hi);
             }
             if (par) {
                 launch_dp_par<DT>(ds.caps, pl, L, level_rows(L - 1), ldh, G[L & 1], ldh, part, nb, st);
@@ -1362,6 +1372,7 @@ static int grad_fit(const DevPlan &pl, int dtype, bool ckpt, int64_t *fit) {
 // pass size within it.
 static int grad_tune(const DevPlan &pl, int dtype, int64_t want, Tune *tune, int64_t *cap) {
     Tune t = pick_tune(pl, dtype, std::max<int64_t>(want, 1));
+    const int64_t fwd_pass = t.chunk;
     int64_t fit;
     EGG_TRY(grad_fit(pl, dtype, t.ckpt, &fit));
     if (!t.ckpt && fit < std::min(want, kGradMinPass)) {
@@ -1379,11 +1390,12 @@ static int grad_tune(const DevPlan &pl, int dtype, int64_t want, Tune *tune, int
                     grad_bytes_per_instance(pl, dtype, true) / 1e9);
     *cap = std::min(want, fit);
     t.chunk = grad_pass(t, want, *cap);
-    // narrow tiles pay off when a pass is at most 8 tiles of 16 (one per XCD); the gradient's
-    // passes are larger, and there the 64-wide kernel with 16 waves per workgroup, 2 per CU
-    // (its best shape at N = 24..32) measured faster (N=28: 17.7 vs 18.1 ms).  The 16-byte-load
-    // kernel keeps its choice.
-    if (t.narrow && !t.vec && t.chunk > 128) {
+    t.sub = fwd_pass > 0 && fwd_pass < t.chunk ? fwd_pass : 0;
+    // the forward part walks sub-passes of the forward's tuned size (t.sub); narrow tiles pay off
+    // when a walk is at most 8 tiles of 16 (one per XCD), so a larger walk takes the 64-wide
+    // kernel with 16 waves per workgroup, 2 per CU instead.  The 16-byte-load kernel keeps its
+    // choice.
+    if (t.narrow && !t.vec && t.chunk > 128 && (t.sub == 0 || t.sub > 128)) {
         t.narrow = false;
         t.wide = true;
         t.occ = 2;
