@@ -99,7 +99,7 @@ def cpu_baseline(K, budget_s=12.0):
     oracle.build_plan(K)
     oracle.haf_dp(P, K, oracle.F64, nthreads)            # warmup
     best, spent, reps = float("inf"), 0.0, 0
-    while spent < budget_s and reps < 100:
+    while spent < budget_s and reps < 200:
         t0 = time.perf_counter()
         oracle.haf_dp(P, K, oracle.F64, nthreads)
         dt = time.perf_counter() - t0

@@ -339,10 +339,14 @@ static int run_pipe(DevState &ds, int64_t nchunks, size_t in_max, size_t out_max
     return EGGMAN_OK;
 }
 
+// Pinned staging per slot and direction, at most (a chunk of a few hundred ms of work at N=28).
+constexpr size_t kPipeSlotBytes = size_t(256) << 20;
+
 // Instances per chunk of a host-entry call: max_batch when the caller gives one; otherwise the
 // whole batch when it is under two passes of the tuned launch, else a quarter of it rounded up
 // to whole passes, so that host staging and transfers overlap compute -- and never more than
-// two slots of per_inst device bytes fit in half of the free device memory.
+// two slots of per_inst device bytes fit in half of the free device memory, nor a slot's pinned
+// staging (per_inst bytes an instance) above kPipeSlotBytes.
 static int host_chunk(int64_t B, int64_t max_batch, int64_t pass, size_t per_inst, int64_t *chunk) {
     if (max_batch > 0) {
         *chunk = std::min(max_batch, B);
@@ -350,11 +354,13 @@ static int host_chunk(int64_t B, int64_t max_batch, int64_t pass, size_t per_ins
     }
     size_t free_b = 0, total_b = 0;
     EGG_HIP(hipMemGetInfo(&free_b, &total_b));
-    const int64_t fit = (int64_t)(free_b / 2 / (2 * std::max<size_t>(per_inst, 1)));
+    per_inst = std::max<size_t>(per_inst, 1);
+    const int64_t fit = (int64_t)(free_b / 2 / (2 * per_inst));
     pass = std::max<int64_t>(pass, 1);
     int64_t want = B;
     if (B >= 2 * pass) want = std::max(pass, ((B + 3) / 4 + pass - 1) / pass * pass);
-    *chunk = std::max<int64_t>(1, std::min(std::min(want, B), fit));
+    const int64_t pin = std::max<int64_t>(pass, (int64_t)(kPipeSlotBytes / per_inst) / pass * pass);
+    *chunk = std::max<int64_t>(1, std::min(std::min(std::min(want, B), fit), pin));
     return EGGMAN_OK;
 }
 
