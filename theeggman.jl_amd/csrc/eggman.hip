@@ -1262,19 +1262,24 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
             return H + (size_t)((gl.kept[L] ? 0 : gl.kept_rows) + gl.off[L]) * ldh;
         };
         // levels L0 .. L1 of the pass, walked in sub-passes of the forward's own tuned size (its
-        // child rows stay in L2 as in the forward alone), each writing its columns of the rows
+        // child rows stay in L2 as in the forward alone), each writing its columns of the rows;
+        // the small levels at either end of the range (at most kSmallLevel states, bound by
+        // latency) are walked once for the whole pass
         const int64_t sub = tune.sub > 0 ? std::min(tune.sub, nb) : nb;
+        auto launch_cols = [&](int L, int64_t s0, int64_t ns) {
+            const char *Ps = Pc + (size_t)s0 * pb;
+            const void *Hc = level_rows(L - 1) + s0;
+            void *Ho = L == top ? (void *)(outc + s0) : (void *)(level_rows(L) + s0);
+            if (!(tune.lds && launch_level_lds<DT, W, 1>(ds.caps, tune, pl, L, Ps, ldp, Hc, ldh, Ho, ldh, ns, st)))
+                launch_level_bt<DT, W>(64, pl, L, Ps, ldp, Hc, ldh, Ho, ldh, ns, st);
+        };
         auto forward_range = [&](int L0, int L1) {
-            for (int64_t s0 = 0; s0 < nb; s0 += sub) {
-                const int64_t ns = std::min(sub, nb - s0);
-                const char *Ps = Pc + (size_t)s0 * pb;
-                for (int L = L0; L <= L1; L++) {
-                    const void *Hc = level_rows(L - 1) + s0;
-                    void *Ho = L == top ? (void *)(outc + s0) : (void *)(level_rows(L) + s0);
-                    if (!(tune.lds && launch_level_lds<DT, W, 1>(ds.caps, tune, pl, L, Ps, ldp, Hc, ldh, Ho, ldh, ns, st)))
-                        launch_level_bt<DT, W>(64, pl, L, Ps, ldp, Hc, ldh, Ho, ldh, ns, st);
-                }
-            }
+            int lb = L0, lt = L1;
+            while (lb <= lt && lay.nst[lb - 1] <= kSmallLevel) launch_cols(lb++, 0, nb);
+            while (lt >= lb && lay.nst[lt - 1] <= kSmallLevel) lt--;
+            for (int64_t s0 = 0; s0 < nb && lb <= lt; s0 += sub)
+                for (int L = lb; L <= lt; L++) launch_cols(L, s0, std::min(sub, nb - s0));
+            for (int L = lt + 1; L <= L1; L++) launch_cols(L, 0, nb);
         };
         k_fill_one<DT><<<dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st>>>(level_rows(1), nb);
         forward_range(2, top);
