@@ -1356,12 +1356,20 @@ static int forward_grad_any(int dtype, DevState &ds, DevPlan &pl, const void *P,
     return fail(EGGMAN_EINVAL, "unknown dtype %d", dtype);
 }
 
-// Instances per pass of the gradient: the forward's tuned pass raised to at least kGradMinPass
-// (the reverse kernels spread a level over (task, 64-instance tile) pairs and want eight or more
-// tiles to fill the device), within the batch and the memory cap.
+// Instances per pass of the gradient: the forward's tuned pass raised to at least
+// grad_min_pass, within the batch and the memory cap.  The reverse kernels spread a level over
+// (task, 64-instance tile) pairs and want eight or more tiles to fill the device (512
+// instances); small degrees want more, about 2^27 state values per pass, since each launch then
+// does too little work: 2048 instances at N=24 take 8.4 ms per 4096 gradients where 512 take
+// 9.3 ms, while N=28 is fastest at 512 (16.9 ms per 1024 against 17.3 ms at 1024).
 constexpr int64_t kGradMinPass = 512;
-static int64_t grad_pass(const Tune &t, int64_t B, int64_t cap) {
-    const int64_t c = std::max<int64_t>(t.chunk > 0 ? t.chunk : B, kGradMinPass);
+static int64_t grad_min_pass(int64_t nstates) {
+    int64_t p = kGradMinPass;
+    while (p * nstates < (int64_t(1) << 27)) p *= 2;
+    return p;
+}
+static int64_t grad_pass(const Tune &t, int64_t nstates, int64_t B, int64_t cap) {
+    const int64_t c = std::max<int64_t>(t.chunk > 0 ? t.chunk : B, grad_min_pass(nstates));
     return std::max<int64_t>(1, std::min(std::min(c, B), cap));
 }
 
@@ -1400,7 +1408,7 @@ static int grad_tune(const DevPlan &pl, int dtype, int64_t want, Tune *tune, int
                     grad_bytes_per_instance(pl, dtype, false) / 1e9,
                     grad_bytes_per_instance(pl, dtype, true) / 1e9);
     *cap = std::min(want, fit);
-    t.chunk = grad_pass(t, want, *cap);
+    t.chunk = grad_pass(t, pl.lay.nstates, want, *cap);
     t.sub = fwd_pass > 0 && fwd_pass < t.chunk ? fwd_pass : 0;
     // the forward part walks sub-passes of the forward's tuned size (t.sub); narrow tiles pay off
     // when a walk is at most 8 tiles of 16 (one per XCD), so a larger walk takes the 64-wide
