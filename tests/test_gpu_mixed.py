@@ -56,3 +56,19 @@ def test_mixed_2048_degrees_16_to_32():
         sample = idx[:2]
         P = synth.pair_batch(np.stack([As[i] for i in sample]))
         assert got[sample].tobytes() == o.haf_dp(P, n, o.F64, 8).tobytes(), n
+
+
+def test_mixed_large_and_small_groups():
+    # groups of 256 or more instances walk on their own and the others share one walk; one call
+    # holding both kinds, the matrices of each degree scattered through the list
+    note("mixed: large and small groups")
+    sizes = np.array([16] * 300 + [20] * 5 + [12] * 9 + [24] * 260 + [8] * 3)
+    np.random.default_rng(44).shuffle(sizes)
+    As = [synth.sym_uniform(int(n), 44, i) for i, n in enumerate(sizes)]
+    got = te.hafnian_mixed(As, check_symmetric=False)
+    for n in (8, 12, 16, 20, 24):
+        idx = np.flatnonzero(sizes == n)
+        same = te.hafnian(np.stack([As[i] for i in idx]), check_symmetric=False)
+        assert got[idx].tobytes() == same.tobytes(), n
+    P = synth.pair_batch(np.stack([As[i] for i in np.flatnonzero(sizes == 20)]))
+    assert got[sizes == 20].tobytes() == o.haf_dp(P, 20, o.F64, 8).tobytes()
