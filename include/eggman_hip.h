@@ -144,8 +144,10 @@ int eggman_dp_forward_repeated(uint64_t plan, int dtype, const void *A_host, int
  * Hafnians of instances of different degrees in one device pass.  Group g has counts[g]
  * instances sharing plan handle plans[g]; P_host is the concatenation of the groups'
  * [pair][counts[g]] blocks (group 0 first) and out_host receives the groups' results in the same
- * order.  Every launch advances each group by one level, so all degrees share the GPU at once.
- * All plans must live on one device and use one index width.  Results are bit-identical to
+ * order.  Groups of 256 or more instances, which fill the GPU by themselves, walk one after
+ * another with their own tuned launch; the smaller groups share one walk in which every launch
+ * advances each of them by one level.  All plans must live on one device and use one index
+ * width.  Results are bit-identical to
  * eggman_dp_forward on each group alone.  New: the reference requires one size per batch
  * (src/hafnian.jl:1277-1280).
  */

@@ -323,8 +323,10 @@ def hafnian_mixed(As, *, check_symmetric: bool = True, backend=None):
     """Hafnians of matrices of different sizes, in input order, in one device pass.
 
     New entry point: the reference's batch form requires one size (src/hafnian.jl:1277-1280).
-    Matrices are grouped by degree; every group shares its plan, and one launch advances all
-    groups by a level.  Each value is bit-identical to ``hafnian`` of that matrix alone.
+    Matrices are grouped by degree and every group shares its plan.  Groups of 256 or more
+    matrices walk one after another with their own tuned launch; the smaller groups share one
+    walk in which each launch advances all of them by a level.  Each value is bit-identical to
+    ``hafnian`` of that matrix alone.
     Degrees above DP_MAX (Int64 layout) and up to it (Int32) run as two passes.
     """
     As = [_as_matrix(A) for A in As]
