@@ -1,4 +1,4 @@
-"""Mixed-degree batches on an MI355X: one device pass over matrices of different sizes, each
+"""Mixed-degree batches on an MI355X: one call over matrices of different sizes, each
 result bit-identical to the single-degree path and to the oracle."""
 import numpy as np
 import pytest
