@@ -66,6 +66,24 @@ def test_other_types_bit_exact(T, N):
     assert got.tobytes() == oracle_vals(As, T).tobytes()
 
 
+@pytest.mark.parametrize("T,N,B", [(np.float64, 22, 1), (np.float64, 26, 1), (np.float64, 30, 1),
+                                   (np.complex128, 26, 1), (np.float32, 28, 2), (np.float64, 26, 5)])
+def test_fused_level_runs_bit_exact(T, N, B):
+    # a one-pass walk of few instances runs its small bottom and top levels as one fused launch
+    # each; the values are the oracle's bits
+    note(f"fused runs {np.dtype(T)} N={N} B={B}")
+    As = stack(N, B, 13, "complex" if np.dtype(T).kind == "c" else "normal").astype(T)
+    got = te.hafnian(As, backend=BE)
+    assert got.tobytes() == oracle_vals(As, T).tobytes()
+
+
+def test_fused_level_runs_exact_integers():
+    G = np.stack([synth.gnp_adjacency(26, 14, b) for b in range(3)]).astype(np.int64)
+    got = te.hafnian_exact(G)
+    want = o.haf_dp(o.gather_pairs_batch(G), 26, o.I128, 8)
+    assert [int(v) for v in got] == [int(v) for v in want]
+
+
 def test_no_fma_on_device():
     x = 1.0 + 2.0**-30
     A = np.zeros((4, 4))

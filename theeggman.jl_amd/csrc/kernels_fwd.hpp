@@ -566,6 +566,71 @@ __global__ __launch_bounds__(1024) void k_fused_lds(
     }
 }
 
+// Levels La .. Lb of each instance in one workgroup, for runs of small levels that would each
+// cost a launch of their own.  Level La reads its children from Hin (level La-1, rows ldh apart),
+// or, when La = 2, from the empty set's one held in LDS; levels La .. Lb-1 are kept in LDS
+// (slot = 0-based plan index - base); level Lb is written to Hout (rows ldo apart).  Instances
+// are spread over workgroups (grid-stride) and each state is still summed by one lane in plan
+// order, so the values are the level kernels' bits.
+template <int DT, typename W>
+__global__ __launch_bounds__(1024) void k_fused_range(
+    const W *__restrict__ trans, FusedLevels lv, int La, int Lb, int base, int nkeep, int npairs,
+    const typename Elt<DT>::P *__restrict__ P, int64_t ldp,
+    const typename Elt<DT>::H *__restrict__ Hin, int64_t ldh,
+    typename Elt<DT>::H *__restrict__ Hout, int64_t ldo, int64_t B) {
+    using E = Elt<DT>;
+    using HT = typename E::H;
+    using PT = typename E::P;
+    constexpr int bits = Packing<W>::bits;
+    constexpr W mask = (W)((W(1) << bits) - 1);
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    HT *Hs = reinterpret_cast<HT *>(lds_raw);
+    PT *Ps = reinterpret_cast<PT *>(lds_raw + (((size_t)nkeep * sizeof(HT) + 15) & ~(size_t)15));
+    const int64_t in_first = La > 2 ? lv.first[La - 2] : 0;   // 0-based plan index of Hin's row 0
+    for (int64_t b = blockIdx.x; b < B; b += gridDim.x) {
+        for (int q = threadIdx.x; q < npairs; q += blockDim.x) Ps[q] = P[(int64_t)q * ldp + b];
+        if (La == 2 && threadIdx.x == 0) Hs[0] = E::one();
+        __syncthreads();
+        for (int L = La; L <= Lb; L++) {
+            const int first = lv.first[L - 1], n = lv.nst[L - 1], k = lv.k[L - 1];
+            const W *tl = trans + lv.tstart[L - 1];
+            const bool global_in = L == La && La > 2;
+            for (int r = threadIdx.x; r < n; r += blockDim.x) {
+                const W *tw = tl + (int64_t)r * k;
+                HT acc = E::zero();
+                if (global_in) {
+                    constexpr int U = 8;   // child loads issued together before they are summed
+                    int i = 0;
+                    for (; i + U <= k; i += U) {
+                        W t[U];
+                        HT h[U];
+#pragma unroll
+                        for (int u = 0; u < U; u++) t[u] = tw[i + u];
+#pragma unroll
+                        for (int u = 0; u < U; u++) h[u] = Hin[((int64_t)(t[u] >> bits) - 1 - in_first) * ldh + b];
+#pragma unroll
+                        for (int u = 0; u < U; u++) acc = E::muladd(acc, Ps[(int)(t[u] & mask) - 1], h[u]);
+                    }
+                    for (; i < k; i++) {
+                        const W t = tw[i];
+                        acc = E::muladd(acc, Ps[(int)(t & mask) - 1],
+                                        Hin[((int64_t)(t >> bits) - 1 - in_first) * ldh + b]);
+                    }
+                } else {
+#pragma unroll 4
+                    for (int i = 0; i < k; i++) {
+                        const W t = tw[i];
+                        acc = E::muladd(acc, Ps[(int)(t & mask) - 1], Hs[(int64_t)(t >> bits) - 1 - base]);
+                    }
+                }
+                if (L == Lb) Hout[(int64_t)r * ldo + b] = acc;
+                else Hs[first + r - base] = acc;
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // Pair values gathered on the device for the boson-sampling batch shape: one matrix A, many
 // index lists (src/dp.jl:479-494).  P[q][b] = A[idx_b[i], idx_b[j]] for the q-th pair i < j in
