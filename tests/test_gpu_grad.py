@@ -121,6 +121,11 @@ def test_gradient_is_deterministic_and_chunk_independent():
     v3, g3 = te._haf_dp_backend_grad(BE, P, te._dp_plan(N), max_batch=7)
     assert v1.tobytes() == v2.tobytes() == v3.tobytes()
     assert np.asarray(g1).tobytes() == np.asarray(g2).tobytes() == np.asarray(g3).tobytes()
+    # passes of 1, 32 and 33 instances: one lane per child, 32 lanes, and the 64-wide kernels
+    for mb in (1, 32, 33):
+        v, g = te._haf_dp_backend_grad(BE, P, te._dp_plan(N), max_batch=mb)
+        assert v.tobytes() == v1.tobytes(), mb
+        assert np.asarray(g).tobytes() == np.asarray(g1).tobytes(), mb
 
 
 def test_repeated_gradient_folds_the_expanded_gradient():

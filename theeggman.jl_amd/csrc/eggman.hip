@@ -1301,6 +1301,25 @@ static void launch_rev_g(const DevCaps &caps, const Tune &tune, const DevPlan &p
     const bool use_lds = tune.lds && lds <= caps.max_lds_block;
     const W *cent = (const W *)pl.d_cent;
     const int64_t *coff = (const int64_t *)pl.d_coff + (lay.levels[L - 2] - 1);
+    if (pow2_at_least(nb) < 64) {
+        // batches of at most 32: as many lanes per child as instances (next power of two)
+        const int bt = pow2_at_least(nb);
+        const int64_t ntiles = (nb + bt - 1) / bt;
+        const int64_t blocks = std::max<int64_t>(
+            1, std::min<int64_t>(kMaxBlocks, (nch * ntiles * bt + kBlock - 1) / kBlock));
+        const auto *Pp = (const typename E::P *)P;
+        const auto *Gsp = (const typename E::H *)Gs;
+        auto *Gcp = (typename E::H *)Gc;
+        switch (bt) {
+        case 1: k_rev_g_bt<DT, W, 1><<<dim3((unsigned)blocks), dim3(kBlock), 0, st>>>(cent, coff, nch, Pp, ldp, Gsp, ldg, Gcp, nb, ntiles); break;
+        case 2: k_rev_g_bt<DT, W, 2><<<dim3((unsigned)blocks), dim3(kBlock), 0, st>>>(cent, coff, nch, Pp, ldp, Gsp, ldg, Gcp, nb, ntiles); break;
+        case 4: k_rev_g_bt<DT, W, 4><<<dim3((unsigned)blocks), dim3(kBlock), 0, st>>>(cent, coff, nch, Pp, ldp, Gsp, ldg, Gcp, nb, ntiles); break;
+        case 8: k_rev_g_bt<DT, W, 8><<<dim3((unsigned)blocks), dim3(kBlock), 0, st>>>(cent, coff, nch, Pp, ldp, Gsp, ldg, Gcp, nb, ntiles); break;
+        case 16: k_rev_g_bt<DT, W, 16><<<dim3((unsigned)blocks), dim3(kBlock), 0, st>>>(cent, coff, nch, Pp, ldp, Gsp, ldg, Gcp, nb, ntiles); break;
+        case 32: k_rev_g_bt<DT, W, 32><<<dim3((unsigned)blocks), dim3(kBlock), 0, st>>>(cent, coff, nch, Pp, ldp, Gsp, ldg, Gcp, nb, ntiles); break;
+        }
+        return;
+    }
     if (use_lds) {
         auto kern = k_rev_g<DT, W, kLdsWaves, true>;
         static bool attr_set = false;
@@ -1370,7 +1389,7 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
             const void *Hc = level_rows(L - 1) + s0;
             void *Ho = L == top ? (void *)(outc + s0) : (void *)(level_rows(L) + s0);
             if (!(tune.lds && launch_level_lds<DT, W, 1>(ds.caps, tune, pl, L, Ps, ldp, Hc, ldh, Ho, ldh, ns, st)))
-                launch_level_bt<DT, W>(64, pl, L, Ps, ldp, Hc, ldh, Ho, ldh, ns, st);
+                launch_level_bt<DT, W>(pow2_at_least(ns), pl, L, Ps, ldp, Hc, ldh, Ho, ldh, ns, st);
         };
         auto forward_range = [&](int L0, int L1) {
             int lb = L0, lt = L1;

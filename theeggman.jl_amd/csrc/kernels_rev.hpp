@@ -109,6 +109,51 @@ __global__ __launch_bounds__(WPB * 64) void k_rev_g(
     }
 }
 
+// The adjoint gather for small batches: BT lanes (BT < 64, a power of two) per (child, tile of BT
+// instances), P read from global memory.  Each lane sums its instance over the child's parent
+// list in list order, as k_rev_g does, so both give the same bits.
+template <int DT, typename W, int BT>
+__global__ __launch_bounds__(kBlock) void k_rev_g_bt(
+    const W *__restrict__ cent, const int64_t *__restrict__ coff, int64_t nch,
+    const typename Elt<DT>::P *__restrict__ P, int64_t ldp,
+    const typename Elt<DT>::H *__restrict__ Gs, int64_t ldg,
+    typename Elt<DT>::H *__restrict__ Gc, int64_t nb, int64_t ntiles) {
+    using E = Elt<DT>;
+    constexpr int bits = Packing<W>::bits;
+    constexpr W mask = (W)((W(1) << bits) - 1);
+    const int lane = (int)(threadIdx.x % BT);
+    const int64_t ngroups = nch * ntiles;
+    const int64_t gstride = (int64_t)gridDim.x * (kBlock / BT);
+    for (int64_t g = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / BT; g < ngroups; g += gstride) {
+        const int64_t c = ntiles == 1 ? g : g / ntiles;
+        const int64_t b = (g - c * ntiles) * BT + lane;
+        if (b >= nb) continue;
+        const int64_t e0 = coff[c], e1 = coff[c + 1];
+        typename E::H acc = E::zero();
+        constexpr int U = 8;   // gathers issued together before they are summed in order
+        int64_t e = e0;
+        for (; e + U <= e1; e += U) {
+            W t[U];
+            typename E::P pv[U];
+            typename E::H gv[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) t[u] = cent[e + u];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                pv[u] = P[((int64_t)(t[u] & mask) - 1) * ldp + b];
+                gv[u] = Gs[(int64_t)(t[u] >> bits) * ldg + b];
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) acc = E::muladd(acc, pv[u], gv[u]);
+        }
+        for (; e < e1; e++) {
+            const W t = cent[e];
+            acc = E::muladd(acc, P[((int64_t)(t & mask) - 1) * ldp + b], Gs[(int64_t)(t >> bits) * ldg + b]);
+        }
+        Gc[c * ldg + b] = acc;
+    }
+}
+
 // Partial dP sums of one level: one wave per (segment, tile), sum over the segment's
 // transitions (parent s, child c) of H[c] * G[s].
 template <int DT>
