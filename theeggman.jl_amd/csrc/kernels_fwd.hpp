@@ -87,6 +87,53 @@ __global__ __launch_bounds__(kBlock) void k_level_fwd(
     }
 }
 
+// One instance, one lane per state, with the instance's pair vector staged in LDS: a lane's pair
+// reads are scattered over the vector, and from LDS they leave the vector memory path to the
+// plan words and the child values.  Each state is summed in plan order, so the values are the
+// other kernels' bits.  P (pair rows ldp apart), Hc and Ho (rows ldh and ldo apart) point at the
+// instance's column.  (Staging the block's plan words in LDS as well, to coalesce their reads,
+// cost more in barriers than it saved.)
+constexpr int kPlainBlock = 256;
+
+template <int DT, typename W>
+__global__ __launch_bounds__(kPlainBlock) void k_level_fwd_plain(
+    const W *__restrict__ trans_lvl, int64_t nst, int k,
+    const typename Elt<DT>::P *__restrict__ P, int64_t ldp, int npairs,
+    const typename Elt<DT>::H *__restrict__ Hc, int64_t cfirst, int64_t ldh,
+    typename Elt<DT>::H *__restrict__ Ho, int64_t ldo) {
+    using E = Elt<DT>;
+    using PT = typename E::P;
+    using HT = typename E::H;
+    constexpr int bits = Packing<W>::bits;
+    constexpr W mask = (W)((W(1) << bits) - 1);
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    PT *Ps = reinterpret_cast<PT *>(lds_raw);
+    for (int q = (int)threadIdx.x; q < npairs; q += kPlainBlock) Ps[q] = P[(int64_t)q * ldp];
+    __syncthreads();
+    for (int64_t s = (int64_t)blockIdx.x * kPlainBlock + threadIdx.x; s < nst;
+         s += (int64_t)gridDim.x * kPlainBlock) {
+        const W *tw = trans_lvl + s * (int64_t)k;
+        HT acc = E::zero();
+        constexpr int U = 8;   // child loads issued together before they are summed in order
+        int i = 0;
+        for (; i + U <= k; i += U) {
+            W t[U];
+            HT h[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) t[u] = tw[i + u];
+#pragma unroll
+            for (int u = 0; u < U; u++) h[u] = Hc[((int64_t)(t[u] >> bits) - cfirst) * ldh];
+#pragma unroll
+            for (int u = 0; u < U; u++) acc = E::muladd(acc, Ps[(int)(t[u] & mask) - 1], h[u]);
+        }
+        for (; i < k; i++) {
+            const W t = tw[i];
+            acc = E::muladd(acc, Ps[(int)(t & mask) - 1], Hc[((int64_t)(t >> bits) - cfirst) * ldh]);
+        }
+        Ho[s * ldo] = acc;
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // Batched level kernel with the level's pair values staged in LDS and an XCD-aware mapping.
 //
