@@ -188,7 +188,10 @@ int eggman_dp_forward_grad_device(uint64_t plan, int dtype, const void *P_dev, i
  * `reps` runs each, and keep the fastest for this (device, degree, layout, dtype, B rounded up
  * to a power of two).  Later calls of the forward entry points use it; a batch size with no entry
  * of its own uses the nearest tuned bucket of the same degree, layout and dtype (the largest
- * below it, else the smallest above).  Every choice gives bit-identical results.  A text report
+ * below it, else the smallest above); a degree or dtype with no entry at all uses defaults that
+ * follow the tuned FP64 choices (the fused LDS walk where it fits, else from 64 instances the
+ * 16-byte-load kernel in passes of about 2^26 state values).  Every choice gives bit-identical
+ * results.  A text report
  * (one line per candidate) is written to `report`.
  * New: the reference has no tuning.
  */

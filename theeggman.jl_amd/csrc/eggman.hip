@@ -427,10 +427,24 @@ static int pow2_at_least(int64_t n) {
     return b;
 }
 
-static Tune default_tune(const PlanLayout &, int, int64_t B) {
+// Launch choices for a (degree, element type) the table does not cover, following what the
+// autotuner picks for the tuned FP64 degrees: the fused LDS-resident walk whenever the plan's
+// states fit in LDS (it declines otherwise); from 64 instances the 16-byte-load LDS kernel with 2
+// workgroups per CU, in passes of about 2^26 state values (512, 128 and 64 instances at N=24,
+// 28 and 32, the tuned passes); below 64 instances, as many lanes per state as instances.
+static Tune default_tune(const PlanLayout &lay, int, int64_t B) {
     Tune t;
     t.bt = pow2_at_least(B);
     t.chunk = B;
+    t.fused = true;
+    if (B >= 64) {
+        t.lds = true;
+        t.vec = true;
+        t.occ = 2;
+        int64_t c = 64;
+        while (c * 2 * lay.nstates <= (int64_t(1) << 26)) c *= 2;
+        t.chunk = std::min(c, B);
+    }
     return t;
 }
 
