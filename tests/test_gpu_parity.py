@@ -228,3 +228,15 @@ def test_device_resident_entry_point():
     te._haf_dp_device(BE, te._dp_plan(K), np.float64, Pd.addr, ldp, B, out.addr, 0)
     hip.device_synchronize()
     assert out.download(np.float64, B).tobytes() == o.haf_dp(P, K, o.F64, 8).tobytes()
+
+
+@pytest.mark.parametrize("T,N,B", [(np.float64, 26, 100), (np.float64, 18, 70), (np.float32, 30, 64),
+                                   (np.complex64, 26, 80), (np.float64, 18, 1)])
+def test_untuned_defaults_bit_exact(T, N, B):
+    # degrees and element types with no tuning entry use the default launch choices (the fused
+    # walk, or the 16-byte-load kernel in passes); the values are the oracle's bits
+    note(f"defaults {np.dtype(T)} N={N} B={B}")
+    assert te.tune_get(N, T, B) is None
+    As = stack(N, B, 15, "complex" if np.dtype(T).kind == "c" else "normal").astype(T)
+    got = te.hafnian(As, backend=BE, check_symmetric=False)
+    assert got.tobytes() == oracle_vals(As, T).tobytes()
