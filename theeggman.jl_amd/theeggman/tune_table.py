@@ -33,6 +33,9 @@ def main():
         for K in (20, 24, 28, 32):
             shapes.append((K, np.complex128, 1))
             shapes.append((K, np.complex128, 256))
+        for K in (24, 28, 32):
+            shapes.append((K, np.float32, 1024))
+            shapes.append((K, np.complex64, 256))
         for K in (36, 40):
             shapes.append((K, np.complex128, 1))
         shapes.append((40, np.int64, 64))
