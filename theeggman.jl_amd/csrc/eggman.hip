@@ -342,12 +342,22 @@ static int run_pipe(DevState &ds, int64_t nchunks, size_t in_max, size_t out_max
 // Pinned staging per slot and direction, at most (a chunk of a few hundred ms of work at N=28).
 constexpr size_t kPipeSlotBytes = size_t(256) << 20;
 
+// Chunks a host-entry call splits into when the caller gives no max_batch: two when the device
+// work per instance is large against its transfer (at least 1000 transitions per pair value,
+// N >= 24), so each chunk's fixed costs (its small levels, launches, the first upload) stay few;
+// four when transfers dominate.  N=28, B=1024: 3.63-3.70 ms in two chunks, 3.72 ms in four;
+// N=24, B=4096: 2.07-2.13 against 2.14 ms.
+static int pipe_parts(const PlanLayout &lay) {
+    return lay.ntrans >= 1000 * lay.npairs ? 2 : 4;
+}
+
 // Instances per chunk of a host-entry call: max_batch when the caller gives one; otherwise the
-// whole batch when it is under two passes of the tuned launch, else a quarter of it rounded up
-// to whole passes, so that host staging and transfers overlap compute -- and never more than
-// two slots of per_inst device bytes fit in half of the free device memory, nor a slot's pinned
+// whole batch when it is under two passes of the tuned launch, else 1/parts of it rounded up to
+// whole passes, so that host staging and transfers overlap compute -- and never more than two
+// slots of per_inst device bytes fit in half of the free device memory, nor a slot's pinned
 // staging (per_inst bytes an instance) above kPipeSlotBytes.
-static int host_chunk(int64_t B, int64_t max_batch, int64_t pass, size_t per_inst, int64_t *chunk) {
+static int host_chunk(int64_t B, int64_t max_batch, int64_t pass, size_t per_inst, int parts,
+                      int64_t *chunk) {
     if (max_batch > 0) {
         *chunk = std::min(max_batch, B);
         return EGGMAN_OK;
@@ -358,7 +368,7 @@ static int host_chunk(int64_t B, int64_t max_batch, int64_t pass, size_t per_ins
     const int64_t fit = (int64_t)(free_b / 2 / (2 * per_inst));
     pass = std::max<int64_t>(pass, 1);
     int64_t want = B;
-    if (B >= 2 * pass) want = std::max(pass, ((B + 3) / 4 + pass - 1) / pass * pass);
+    if (B >= 2 * pass) want = std::max(pass, ((B + parts - 1) / parts + pass - 1) / pass * pass);
     const int64_t pin = std::max<int64_t>(pass, (int64_t)(kPipeSlotBytes / per_inst) / pass * pass);
     *chunk = std::max<int64_t>(1, std::min(std::min(std::min(want, B), fit), pin));
     return EGGMAN_OK;
@@ -1789,7 +1799,8 @@ int eggman_dp_forward(uint64_t plan, int dtype, const void *P_host, int64_t B, i
     EGG_TRY(dev_state(pl->device, &ds));
     const Tune t0 = pick_tune(*pl, dtype, B);
     int64_t chunk;
-    EGG_TRY(host_chunk(B, max_batch, t0.chunk > 0 ? t0.chunk : B, (size_t)npairs * pb + hb, &chunk));
+    EGG_TRY(host_chunk(B, max_batch, t0.chunk > 0 ? t0.chunk : B, (size_t)npairs * pb + hb,
+                       pipe_parts(pl->lay), &chunk));
     const Tune tune = pick_tune(*pl, dtype, chunk);
     const int64_t nchunks = (B + chunk - 1) / chunk;
     auto span = [&](int64_t c, int64_t *b0, int64_t *nb) {
@@ -2032,7 +2043,8 @@ int eggman_dp_forward_repeated(uint64_t plan, int dtype, const void *A_host, int
     const Tune t0 = pick_tune(*pl, dtype, B);
     int64_t chunk;
     EGG_TRY(host_chunk(B, max_batch, t0.chunk > 0 ? t0.chunk : B,
-                       (size_t)K * sizeof(int32_t) + hb + (size_t)npairs * pb / 2, &chunk));
+                       (size_t)K * sizeof(int32_t) + hb + (size_t)npairs * pb / 2,
+                       pipe_parts(pl->lay), &chunk));
     const Tune tune = pick_tune(*pl, dtype, chunk);
     const int64_t nchunks = (B + chunk - 1) / chunk;
     // the matrix once, then the pair values of each chunk gathered on the device into one buffer
@@ -2142,7 +2154,7 @@ int eggman_dp_forward_grad(uint64_t plan, int dtype, const void *P_host, int64_t
     Tune tune;
     int64_t cap, chunk;
     EGG_TRY(grad_tune(*pl, dtype, max_batch > 0 ? std::min(max_batch, B) : B, &tune, &cap));
-    EGG_TRY(host_chunk(B, max_batch, tune.chunk, (size_t)npairs * (pb + hb) + hb, &chunk));
+    EGG_TRY(host_chunk(B, max_batch, tune.chunk, (size_t)npairs * (pb + hb) + hb, 2, &chunk));
     chunk = std::min(chunk, cap);
     const int64_t nchunks = (B + chunk - 1) / chunk;
     auto span = [&](int64_t c, int64_t *b0, int64_t *nb) {
