@@ -1321,13 +1321,38 @@ static void launch_dp_par_w(const DevCaps &caps, const DevPlan &pl, int L, int r
     }
 }
 
+// One instance: k_rev_dp_par_one with the same slicing (WPB slices of the task) as the 64-wide
+// kernel, so the same bits.
+template <int DT, int WPB>
+static void launch_dp_par_one(const DevPlan &pl, int L, int rows, const void *Hc, int64_t ldh,
+                              const void *Gs, int64_t ldg, void *part, hipStream_t st) {
+    using HT = typename Elt<DT>::H;
+    const PlanLayout &lay = pl.lay;
+    const int64_t t0 = pl.lvl_task[L - 1], nt = pl.lvl_task[L] - t0;
+    if (nt == 0) return;
+    const size_t lds = (size_t)WPB * rows * sizeof(HT);
+    k_rev_dp_par_one<DT, WPB><<<dim3((unsigned)std::min<int64_t>(kMaxBlocks, nt)), dim3(WPB * 32), lds, st>>>(
+        (const int32_t *)pl.d_trans + lay.tstart[L - 1], lay.size[L - 1] - 1, lay.K,
+        (const int32_t *)pl.d_prank, (const ParTaskDev *)pl.d_tasks, t0, nt, (const HT *)Hc,
+        lay.levels[L - 2], ldh, (const HT *)Gs, ldg, (HT *)part, 1, rows,
+        (kParTaskLen + WPB - 1) / WPB);
+}
+
 template <int DT>
 static void launch_dp_par(const DevCaps &caps, const DevPlan &pl, int L, const void *Hc,
                           int64_t ldh, const void *Gs, int64_t ldg, void *part, int64_t nb,
                           hipStream_t st) {
     int rows;
     size_t lds;
-    switch (par_waves(caps, pl.lay, L, DT, &rows, &lds)) {
+    const int wpb = par_waves(caps, pl.lay, L, DT, &rows, &lds);
+    if (nb == 1 && pl.lay.size[L - 1] - 1 <= 32) {
+        switch (wpb) {
+        case 8: launch_dp_par_one<DT, 8>(pl, L, rows, Hc, ldh, Gs, ldg, part, st); return;
+        case 4: launch_dp_par_one<DT, 4>(pl, L, rows, Hc, ldh, Gs, ldg, part, st); return;
+        case 2: launch_dp_par_one<DT, 2>(pl, L, rows, Hc, ldh, Gs, ldg, part, st); return;
+        }
+    }
+    switch (wpb) {
     case 8: launch_dp_par_w<DT, 8>(caps, pl, L, rows, lds, Hc, ldh, Gs, ldg, part, nb, st); break;
     case 4: launch_dp_par_w<DT, 4>(caps, pl, L, rows, lds, Hc, ldh, Gs, ldg, part, nb, st); break;
     case 2: launch_dp_par_w<DT, 2>(caps, pl, L, rows, lds, Hc, ldh, Gs, ldg, part, nb, st); break;

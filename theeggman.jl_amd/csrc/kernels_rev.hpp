@@ -338,6 +338,59 @@ __global__ __launch_bounds__(WPB * 64) void k_rev_dp_par(
     }
 }
 
+// Parent-major dP for one instance: the sums of k_rev_dp_par in the same order, with lanes
+// over a parent's transitions instead of over instances.  Slice w of a task (parents
+// [w * sub_len, (w + 1) * sub_len), as wave w of k_rev_dp_par takes them) runs on 32 lanes, lane t
+// holding transition t of the current parent (k <= 31 on the Int32 layout); a parent's
+// transitions have distinct partners, so its lanes update distinct slots of the slice's LDS row,
+// parent after parent.  The slices' rows are then added in slice order.  Every slot therefore
+// sums the same terms in the same order as the 64-wide kernel, without 63 idle lanes per wave
+// and with a few hundred bytes of LDS per task, so many tasks share a CU.
+template <int DT, int WPB>
+__global__ __launch_bounds__(WPB * 32) void k_rev_dp_par_one(
+    const int32_t *__restrict__ trans_lvl, int k, int K, const int32_t *__restrict__ prank,
+    const ParTaskDev *__restrict__ tasks, int64_t task0, int64_t ntask,
+    const typename Elt<DT>::H *__restrict__ Hc, int64_t cfirst, int64_t ldh,
+    const typename Elt<DT>::H *__restrict__ Gs, int64_t ldg, typename Elt<DT>::H *__restrict__ part,
+    int64_t nb, int rows_per_slice, int sub_len) {
+    using E = Elt<DT>;
+    using HT = typename E::H;
+    constexpr int bits = Packing<int32_t>::bits;
+    constexpr int32_t mask = (1 << bits) - 1;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    HT *rows = reinterpret_cast<HT *>(lds_raw);
+    const int t = (int)(threadIdx.x & 31);
+    const int slice = (int)(threadIdx.x >> 5);
+    HT *acc = rows + slice * rows_per_slice;
+    for (int64_t ti = blockIdx.x; ti < ntask; ti += gridDim.x) {
+        const ParTaskDev tk = tasks[task0 + ti];
+        const int nrow = K - tk.p;
+        const int qbase = (tk.p - 1) * K - tk.p * (tk.p + 1) / 2 + tk.p;
+        for (int u = t; u < nrow; u += 32) acc[u] = E::zero();
+        const int i0 = slice * sub_len;
+        const int cnt = min(tk.len - i0, sub_len);
+        for (int j = 0; j < cnt; j++) {
+            const int r = prank[tk.start + i0 + j];
+            const HT g = Gs[(int64_t)r * ldg];
+            if (t < k) {
+                const int32_t wd = trans_lvl[(int64_t)r * k + t];
+                const HT h = Hc[((int64_t)(wd >> bits) - cfirst) * ldh];
+                const int slot = (wd & mask) - 1 - qbase;
+                acc[slot] = E::muladd_hh(acc[slot], h, g);
+            }
+        }
+        __syncthreads();
+        // the task's row: the slices' rows summed in slice order
+        for (int u = (int)threadIdx.x; u < nrow; u += WPB * 32) {
+            HT sum = rows[u];
+#pragma unroll
+            for (int v = 1; v < WPB; v++) sum = E::add_h(sum, rows[v * rows_per_slice + u]);
+            part[(tk.row0 + u) * nb] = sum;
+        }
+        __syncthreads();
+    }
+}
+
 // Sum of `n` values src[idx(i)] in ascending i, loads issued eight at a time.
 template <int DT, typename F>
 __device__ __forceinline__ typename Elt<DT>::H ordered_sum(int64_t n, F &&load) {
