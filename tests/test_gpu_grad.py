@@ -112,8 +112,10 @@ def test_config4_batched_gradient_n24():
     _ = rng
 
 
-def test_gradient_is_deterministic_and_chunk_independent():
-    N, B = 16, 70
+@pytest.mark.parametrize("N", [16, 24])
+def test_gradient_is_deterministic_and_chunk_independent(N):
+    B = 70
+    note(f"grad chunks N={N}")
     As = np.stack([synth.sym_uniform(N, 31, b) for b in range(B)])
     P = te._gather_pairs_batch(np.float64, As, None, N)
     v1, g1 = te._haf_dp_backend_grad(BE, P, te._dp_plan(N))
