@@ -474,27 +474,30 @@ __device__ inline void level_lds_vec_body_a(const LevelArgs<DT, W> &a, int64_t j
         HT acc[VEC];
 #pragma unroll
         for (int v = 0; v < VEC; v++) acc[v] = E::zero();
-        int i = 0;
-        for (; i + U <= a.k; i += U) {
-            W t[U];
-            HV h[U];
+        // N transitions from i: their N child loads issued together, then summed in order
+        auto batch = [&](int i, auto n) {
+            constexpr int N = decltype(n)::value;
+            W t[N];
+            HV h[N];
 #pragma unroll
-            for (int u = 0; u < U; u++) t[u] = word(i + u);
+            for (int u = 0; u < N; u++) t[u] = word(i + u);
 #pragma unroll
-            for (int u = 0; u < U; u++) h[u] = load_child(t[u]);
+            for (int u = 0; u < N; u++) h[u] = load_child(t[u]);
 #pragma unroll
-            for (int u = 0; u < U; u++) {
+            for (int u = 0; u < N; u++) {
                 const PV p = pair_val(t[u]);
 #pragma unroll
                 for (int v = 0; v < VEC; v++) acc[v] = E::muladd(acc[v], p.v[v], h[u].v[v]);
             }
-        }
-        for (; i < a.k; i++) {
-            const W t = word(i);
-            const HV h = load_child(t);
-            const PV p = pair_val(t);
-#pragma unroll
-            for (int v = 0; v < VEC; v++) acc[v] = E::muladd(acc[v], p.v[v], h.v[v]);
+        };
+        int i = 0;
+        for (; i + U <= a.k; i += U) batch(i, std::integral_constant<int, U>{});
+        // the last k mod U transitions as one batch too (k is uniform, so is the branch)
+        switch (a.k - i) {
+        case 3: batch(i, std::integral_constant<int, 3>{}); break;
+        case 2: batch(i, std::integral_constant<int, 2>{}); break;
+        case 1: batch(i, std::integral_constant<int, 1>{}); break;
+        default: break;
         }
         if (ok) {
 #pragma unroll
