@@ -233,24 +233,28 @@ __device__ inline void level_lds_body(const LevelArgs<DT, W> &a, int64_t j,
         const W *tw = a.trans_lvl + s * (int64_t)a.k;
         typename E::H acc = E::zero();
         if (live) {
-            int i = 0;
-            for (; i + U <= a.k; i += U) {
-                W t[U];
-                typename E::H h[U];
+            // N transitions from i: their N child loads issued together, then summed in order
+            auto batch = [&](int i, auto n) {
+                constexpr int N = decltype(n)::value;
+                W t[N];
+                typename E::H h[N];
 #pragma unroll
-                for (int u = 0; u < U; u++) t[u] = tw[i + u];
+                for (int u = 0; u < N; u++) t[u] = tw[i + u];
 #pragma unroll
-                for (int u = 0; u < U; u++)
+                for (int u = 0; u < N; u++)
                     h[u] = a.Hc[((int64_t)(t[u] >> bits) - a.cfirst) * a.ldh + b];
 #pragma unroll
-                for (int u = 0; u < U; u++)
+                for (int u = 0; u < N; u++)
                     acc = E::muladd(acc, lds[((int)(t[u] & mask) - 1 - a.qlo) * 64 + lane], h[u]);
-            }
-            for (; i < a.k; i++) {
-                const W t = tw[i];
-                const int q = (int)(t & mask) - 1 - a.qlo;
-                const int64_t c = (int64_t)(t >> bits) - a.cfirst;
-                acc = E::muladd(acc, lds[q * 64 + lane], a.Hc[c * a.ldh + b]);
+            };
+            int i = 0;
+            for (; i + U <= a.k; i += U) batch(i, std::integral_constant<int, U>{});
+            // the last k mod U transitions as one batch too
+            switch (a.k - i) {
+            case 3: batch(i, std::integral_constant<int, 3>{}); break;
+            case 2: batch(i, std::integral_constant<int, 2>{}); break;
+            case 1: batch(i, std::integral_constant<int, 1>{}); break;
+            default: break;
             }
             a.Ho[s * a.ldo + b] = acc;
         }
@@ -345,20 +349,26 @@ __device__ inline void level_lds_narrow_body_a(const LevelArgs<DT, W> &a, int64_
             return *reinterpret_cast<const PT *>(lbase + q * (uint32_t)(TW * sizeof(PT)));
         };
         HT acc = E::zero();
+        // N transitions from i: their N child loads issued together, then summed in order
+        auto batch = [&](int i, auto n) {
+            constexpr int N = decltype(n)::value;
+            W t[N];
+            HT h[N];
+#pragma unroll
+            for (int u = 0; u < N; u++) t[u] = word(i + u);
+#pragma unroll
+            for (int u = 0; u < N; u++) h[u] = load_child(t[u]);
+#pragma unroll
+            for (int u = 0; u < N; u++) acc = E::muladd(acc, pair_val(t[u]), h[u]);
+        };
         int i = 0;
-        for (; i + U <= a.k; i += U) {
-            W t[U];
-            HT h[U];
-#pragma unroll
-            for (int u = 0; u < U; u++) t[u] = word(i + u);
-#pragma unroll
-            for (int u = 0; u < U; u++) h[u] = load_child(t[u]);
-#pragma unroll
-            for (int u = 0; u < U; u++) acc = E::muladd(acc, pair_val(t[u]), h[u]);
-        }
-        for (; i < a.k; i++) {
-            const W t = word(i);
-            acc = E::muladd(acc, pair_val(t), load_child(t));
+        for (; i + U <= a.k; i += U) batch(i, std::integral_constant<int, U>{});
+        // the last k mod U transitions as one batch too
+        switch (a.k - i) {
+        case 3: batch(i, std::integral_constant<int, 3>{}); break;
+        case 2: batch(i, std::integral_constant<int, 2>{}); break;
+        case 1: batch(i, std::integral_constant<int, 1>{}); break;
+        default: break;
         }
         if (ok && live) a.Ho[s * a.ldo + b] = acc;
     }

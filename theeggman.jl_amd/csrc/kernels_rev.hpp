@@ -82,27 +82,30 @@ __global__ __launch_bounds__(WPB * 64) void k_rev_g(
         typename E::H acc = E::zero();
         if (live) {
             constexpr int U = 4;   // gathers issued together before they are summed in order
-            int64_t e = e0;
-            for (; e + U <= e1; e += U) {
-                W t[U];
-                typename E::H g[U];
+            // N entries from e: their N gathers issued together, then summed in list order
+            auto batch = [&](int64_t e, auto n) {
+                constexpr int N = decltype(n)::value;
+                W t[N];
+                typename E::H g[N];
 #pragma unroll
-                for (int u = 0; u < U; u++) t[u] = cent[e + u];
+                for (int u = 0; u < N; u++) t[u] = cent[e + u];
 #pragma unroll
-                for (int u = 0; u < U; u++) g[u] = Gs[(int64_t)(t[u] >> bits) * ldg + b];
+                for (int u = 0; u < N; u++) g[u] = Gs[(int64_t)(t[u] >> bits) * ldg + b];
 #pragma unroll
-                for (int u = 0; u < U; u++) {
+                for (int u = 0; u < N; u++) {
                     const int q = (int)(t[u] & mask) - 1 - qlo;
                     const PT pv = USE_LDS ? lds[q * 64 + lane] : P[(int64_t)(qlo + q) * ldp + b];
                     acc = E::muladd(acc, pv, g[u]);
                 }
-            }
-            for (; e < e1; e++) {
-                const W t = cent[e];
-                const int q = (int)(t & mask) - 1 - qlo;
-                const int64_t s = (int64_t)(t >> bits);
-                const PT pv = USE_LDS ? lds[q * 64 + lane] : P[(int64_t)(qlo + q) * ldp + b];
-                acc = E::muladd(acc, pv, Gs[s * ldg + b]);
+            };
+            int64_t e = e0;
+            for (; e + U <= e1; e += U) batch(e, std::integral_constant<int, U>{});
+            // the last entries (list length mod U) as one batch too
+            switch ((int)(e1 - e)) {
+            case 3: batch(e, std::integral_constant<int, 3>{}); break;
+            case 2: batch(e, std::integral_constant<int, 2>{}); break;
+            case 1: batch(e, std::integral_constant<int, 1>{}); break;
+            default: break;
             }
             Gc[c * ldg + b] = acc;
         }
