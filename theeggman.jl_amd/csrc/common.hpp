@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 #include <stdarg.h>
 #include <string>
 
@@ -33,6 +34,17 @@ int fail(int code, const char *fmt, ...);
 // fp-contract off, so no fused multiply-add is formed and a state's sum matches the reference's
 // _dp_states! (src/dp.jl:344-363) bit for bit when accumulated in plan order.  Complex multiply
 // is (ac - bd, ad + bc), Julia's `*` for Complex.
+
+// The tail of a gather loop taken in batches of U: `batch(n)` with n an integral_constant equal to
+// rem (0 < rem < U), so the last rem loads are issued together like a full batch's instead of one
+// round trip each.
+template <int U, class F>
+__device__ inline void batch_tail(int rem, F &&batch) {
+    if constexpr (U > 1) {
+        if (rem == U - 1) batch(std::integral_constant<int, U - 1>{});
+        else batch_tail<U - 1>(rem, batch);
+    }
+}
 
 template <int DT> struct Elt;
 

@@ -62,27 +62,24 @@ __global__ __launch_bounds__(kBlock) void k_level_fwd(
         const W *tw = trans_lvl + s * (int64_t)k;
         typename E::H acc = E::zero();
         constexpr int U = 8;   // loads issued together before they are summed in order
-        int i = 0;
-        for (; i + U <= k; i += U) {
-            W t[U];
-            typename E::P pv[U];
-            typename E::H hv[U];
+        auto batch = [&](int i, auto n) {
+            constexpr int N = decltype(n)::value;
+            W t[N];
+            typename E::P pv[N];
+            typename E::H hv[N];
 #pragma unroll
-            for (int u = 0; u < U; u++) t[u] = tw[i + u];
+            for (int u = 0; u < N; u++) t[u] = tw[i + u];
 #pragma unroll
-            for (int u = 0; u < U; u++) {
+            for (int u = 0; u < N; u++) {
                 pv[u] = P[((int64_t)(t[u] & mask) - 1) * ldp + b];
                 hv[u] = Hc[((int64_t)(t[u] >> bits) - cfirst) * ldh + b];
             }
 #pragma unroll
-            for (int u = 0; u < U; u++) acc = E::muladd(acc, pv[u], hv[u]);
-        }
-        for (; i < k; i++) {
-            const W t = tw[i];
-            const int64_t q = (int64_t)(t & mask) - 1;
-            const int64_t c = (int64_t)(t >> bits) - cfirst;
-            acc = E::muladd(acc, P[q * ldp + b], Hc[c * ldh + b]);
-        }
+            for (int u = 0; u < N; u++) acc = E::muladd(acc, pv[u], hv[u]);
+        };
+        int i = 0;
+        for (; i + U <= k; i += U) batch(i, std::integral_constant<int, U>{});
+        batch_tail<U>(k - i, [&](auto n) { batch(i, n); });
         Ho[s * ldo + b] = acc;
     }
 }
@@ -115,21 +112,20 @@ __global__ __launch_bounds__(kPlainBlock) void k_level_fwd_plain(
         const W *tw = trans_lvl + s * (int64_t)k;
         HT acc = E::zero();
         constexpr int U = 8;   // child loads issued together before they are summed in order
+        auto batch = [&](int i, auto n) {
+            constexpr int N = decltype(n)::value;
+            W t[N];
+            HT h[N];
+#pragma unroll
+            for (int u = 0; u < N; u++) t[u] = tw[i + u];
+#pragma unroll
+            for (int u = 0; u < N; u++) h[u] = Hc[((int64_t)(t[u] >> bits) - cfirst) * ldh];
+#pragma unroll
+            for (int u = 0; u < N; u++) acc = E::muladd(acc, Ps[(int)(t[u] & mask) - 1], h[u]);
+        };
         int i = 0;
-        for (; i + U <= k; i += U) {
-            W t[U];
-            HT h[U];
-#pragma unroll
-            for (int u = 0; u < U; u++) t[u] = tw[i + u];
-#pragma unroll
-            for (int u = 0; u < U; u++) h[u] = Hc[((int64_t)(t[u] >> bits) - cfirst) * ldh];
-#pragma unroll
-            for (int u = 0; u < U; u++) acc = E::muladd(acc, Ps[(int)(t[u] & mask) - 1], h[u]);
-        }
-        for (; i < k; i++) {
-            const W t = tw[i];
-            acc = E::muladd(acc, Ps[(int)(t & mask) - 1], Hc[((int64_t)(t >> bits) - cfirst) * ldh]);
-        }
+        for (; i + U <= k; i += U) batch(i, std::integral_constant<int, U>{});
+        batch_tail<U>(k - i, [&](auto n) { batch(i, n); });
         Ho[s * ldo] = acc;
     }
 }
@@ -250,12 +246,7 @@ __device__ inline void level_lds_body(const LevelArgs<DT, W> &a, int64_t j,
             int i = 0;
             for (; i + U <= a.k; i += U) batch(i, std::integral_constant<int, U>{});
             // the last k mod U transitions as one batch too
-            switch (a.k - i) {
-            case 3: batch(i, std::integral_constant<int, 3>{}); break;
-            case 2: batch(i, std::integral_constant<int, 2>{}); break;
-            case 1: batch(i, std::integral_constant<int, 1>{}); break;
-            default: break;
-            }
+            batch_tail<U>(a.k - i, [&](auto n) { batch(i, n); });
             a.Ho[s * a.ldo + b] = acc;
         }
     }
@@ -364,12 +355,7 @@ __device__ inline void level_lds_narrow_body_a(const LevelArgs<DT, W> &a, int64_
         int i = 0;
         for (; i + U <= a.k; i += U) batch(i, std::integral_constant<int, U>{});
         // the last k mod U transitions as one batch too
-        switch (a.k - i) {
-        case 3: batch(i, std::integral_constant<int, 3>{}); break;
-        case 2: batch(i, std::integral_constant<int, 2>{}); break;
-        case 1: batch(i, std::integral_constant<int, 1>{}); break;
-        default: break;
-        }
+        batch_tail<U>(a.k - i, [&](auto n) { batch(i, n); });
         if (ok && live) a.Ho[s * a.ldo + b] = acc;
     }
 }
@@ -503,12 +489,7 @@ __device__ inline void level_lds_vec_body_a(const LevelArgs<DT, W> &a, int64_t j
         int i = 0;
         for (; i + U <= a.k; i += U) batch(i, std::integral_constant<int, U>{});
         // the last k mod U transitions as one batch too (k is uniform, so is the branch)
-        switch (a.k - i) {
-        case 3: batch(i, std::integral_constant<int, 3>{}); break;
-        case 2: batch(i, std::integral_constant<int, 2>{}); break;
-        case 1: batch(i, std::integral_constant<int, 1>{}); break;
-        default: break;
-        }
+        batch_tail<U>(a.k - i, [&](auto n) { batch(i, n); });
         if (ok) {
 #pragma unroll
             for (int v = 0; v < VEC; v++)
@@ -660,22 +641,20 @@ __global__ __launch_bounds__(1024) void k_fused_range(
                 HT acc = E::zero();
                 if (global_in) {
                     constexpr int U = 8;   // child loads issued together before they are summed
+                    auto batch = [&](int i, auto n) {
+                        constexpr int N = decltype(n)::value;
+                        W t[N];
+                        HT h[N];
+#pragma unroll
+                        for (int u = 0; u < N; u++) t[u] = tw[i + u];
+#pragma unroll
+                        for (int u = 0; u < N; u++) h[u] = Hin[((int64_t)(t[u] >> bits) - 1 - in_first) * ldh + b];
+#pragma unroll
+                        for (int u = 0; u < N; u++) acc = E::muladd(acc, Ps[(int)(t[u] & mask) - 1], h[u]);
+                    };
                     int i = 0;
-                    for (; i + U <= k; i += U) {
-                        W t[U];
-                        HT h[U];
-#pragma unroll
-                        for (int u = 0; u < U; u++) t[u] = tw[i + u];
-#pragma unroll
-                        for (int u = 0; u < U; u++) h[u] = Hin[((int64_t)(t[u] >> bits) - 1 - in_first) * ldh + b];
-#pragma unroll
-                        for (int u = 0; u < U; u++) acc = E::muladd(acc, Ps[(int)(t[u] & mask) - 1], h[u]);
-                    }
-                    for (; i < k; i++) {
-                        const W t = tw[i];
-                        acc = E::muladd(acc, Ps[(int)(t & mask) - 1],
-                                        Hin[((int64_t)(t >> bits) - 1 - in_first) * ldh + b]);
-                    }
+                    for (; i + U <= k; i += U) batch(i, std::integral_constant<int, U>{});
+                    batch_tail<U>(k - i, [&](auto n) { batch(i, n); });
                 } else {
 #pragma unroll 4
                     for (int i = 0; i < k; i++) {

@@ -101,12 +101,7 @@ __global__ __launch_bounds__(WPB * 64) void k_rev_g(
             int64_t e = e0;
             for (; e + U <= e1; e += U) batch(e, std::integral_constant<int, U>{});
             // the last entries (list length mod U) as one batch too
-            switch ((int)(e1 - e)) {
-            case 3: batch(e, std::integral_constant<int, 3>{}); break;
-            case 2: batch(e, std::integral_constant<int, 2>{}); break;
-            case 1: batch(e, std::integral_constant<int, 1>{}); break;
-            default: break;
-            }
+            batch_tail<U>((int)(e1 - e), [&](auto n) { batch(e, n); });
             Gc[c * ldg + b] = acc;
         }
     }
@@ -134,25 +129,24 @@ __global__ __launch_bounds__(kBlock) void k_rev_g_bt(
         const int64_t e0 = coff[c], e1 = coff[c + 1];
         typename E::H acc = E::zero();
         constexpr int U = 8;   // gathers issued together before they are summed in order
-        int64_t e = e0;
-        for (; e + U <= e1; e += U) {
-            W t[U];
-            typename E::P pv[U];
-            typename E::H gv[U];
+        auto batch = [&](int64_t e, auto n) {
+            constexpr int N = decltype(n)::value;
+            W t[N];
+            typename E::P pv[N];
+            typename E::H gv[N];
 #pragma unroll
-            for (int u = 0; u < U; u++) t[u] = cent[e + u];
+            for (int u = 0; u < N; u++) t[u] = cent[e + u];
 #pragma unroll
-            for (int u = 0; u < U; u++) {
+            for (int u = 0; u < N; u++) {
                 pv[u] = P[((int64_t)(t[u] & mask) - 1) * ldp + b];
                 gv[u] = Gs[(int64_t)(t[u] >> bits) * ldg + b];
             }
 #pragma unroll
-            for (int u = 0; u < U; u++) acc = E::muladd(acc, pv[u], gv[u]);
-        }
-        for (; e < e1; e++) {
-            const W t = cent[e];
-            acc = E::muladd(acc, P[((int64_t)(t & mask) - 1) * ldp + b], Gs[(int64_t)(t >> bits) * ldg + b]);
-        }
+            for (int u = 0; u < N; u++) acc = E::muladd(acc, pv[u], gv[u]);
+        };
+        int64_t e = e0;
+        for (; e + U <= e1; e += U) batch(e, std::integral_constant<int, U>{});
+        batch_tail<U>((int)(e1 - e), [&](auto n) { batch(e, n); });
         Gc[c * ldg + b] = acc;
     }
 }
