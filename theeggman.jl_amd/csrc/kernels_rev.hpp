@@ -173,23 +173,21 @@ __global__ __launch_bounds__(256) void k_rev_dp_seg(
         const uint64_t *pe = pent + sg.start;
         typename E::H acc = E::zero();
         constexpr int U = 8;   // gathers issued together before they are summed in order
-        int i = 0;
-        for (; i + U <= sg.len; i += U) {
-            typename E::H hv[U], gv[U];
+        auto batch = [&](int i, auto n) {
+            constexpr int N = decltype(n)::value;
+            typename E::H hv[N], gv[N];
 #pragma unroll
-            for (int u = 0; u < U; u++) {
+            for (int u = 0; u < N; u++) {
                 const uint64_t w = pe[i + u];
                 hv[u] = Hc[(int64_t)(w & 0xffffffffull) * ldh + b];
                 gv[u] = Gs[(int64_t)(w >> 32) * ldg + b];
             }
 #pragma unroll
-            for (int u = 0; u < U; u++) acc = E::muladd_hh(acc, hv[u], gv[u]);
-        }
-        for (; i < sg.len; i++) {
-            const uint64_t w = pe[i];
-            const int64_t s = (int64_t)(w >> 32), c = (int64_t)(w & 0xffffffffull);
-            acc = E::muladd_hh(acc, Hc[c * ldh + b], Gs[s * ldg + b]);
-        }
+            for (int u = 0; u < N; u++) acc = E::muladd_hh(acc, hv[u], gv[u]);
+        };
+        int i = 0;
+        for (; i + U <= sg.len; i += U) batch(i, std::integral_constant<int, U>{});
+        batch_tail<U>(sg.len - i, [&](auto n) { batch(i, n); });
         part[(seg0 + si) * nb + b] = acc;
     }
 }
@@ -388,20 +386,23 @@ __global__ __launch_bounds__(WPB * 32) void k_rev_dp_par_one(
     }
 }
 
-// Sum of `n` values src[idx(i)] in ascending i, loads issued eight at a time.
+// Sum of `n` values src[idx(i)] in ascending i, loads issued eight at a time (the tail as one
+// batch).
 template <int DT, typename F>
 __device__ __forceinline__ typename Elt<DT>::H ordered_sum(int64_t n, F &&load) {
     using E = Elt<DT>;
     typename E::H acc = E::zero();
     int64_t i = 0;
-    for (; i + 8 <= n; i += 8) {
-        typename E::H v[8];
+    auto batch = [&](int64_t i, auto m) {
+        constexpr int N = decltype(m)::value;
+        typename E::H v[N];
 #pragma unroll
-        for (int j = 0; j < 8; j++) v[j] = load(i + j);
+        for (int j = 0; j < N; j++) v[j] = load(i + j);
 #pragma unroll
-        for (int j = 0; j < 8; j++) acc = E::add_h(acc, v[j]);
-    }
-    for (; i < n; i++) acc = E::add_h(acc, load(i));
+        for (int j = 0; j < N; j++) acc = E::add_h(acc, v[j]);
+    };
+    for (; i + 8 <= n; i += 8) batch(i, std::integral_constant<int, 8>{});
+    batch_tail<8>((int)(n - i), [&](auto m) { batch(i, m); });
     return acc;
 }
 
