@@ -213,6 +213,8 @@ int eggman_autotune(uint64_t plan, int dtype, int64_t B, int reps, char *report,
 #define EGGMAN_TUNE_VEC 64    /* LDS kernel with 16-byte loads: 16 / sizeof(T) instances per
                                  lane, 8 lanes per state, 8 states per wave (level rows padded
                                  to whole 128-byte tile rows) */
+#define EGGMAN_TUNE_DEEP 128  /* with EGGMAN_TUNE_VEC: child loads in batches of 6 transitions
+                                 instead of 4 */
 
 /*
  * Set / read a tuning entry directly (e.g. to persist tuned choices across processes).  bt:

@@ -402,19 +402,20 @@ struct Tune {
     bool ckpt = false;  // gradient: keep checkpoint levels only and recompute the others
     bool narrow = false;// LDS kernel with 16-instance tiles, 4 states per wave
     bool vec = false;   // LDS kernel with 16-byte loads: 8 lanes per state, 8 states per wave
+    bool deep = false;  // with vec: child loads in batches of 6 transitions instead of 4
     int64_t sub = 0;    // gradient: instances per forward sub-pass inside a pass (0 = the pass)
 };
 
 // Tuning flags as they cross the C ABI (eggman_tune_set / eggman_tune_get).
 enum {
     TUNE_LDS = 1, TUNE_GRAPH = 2, TUNE_FUSED = 4, TUNE_WIDE = 8, TUNE_CKPT = 16, TUNE_NARROW = 32,
-    TUNE_VEC = 64,
-    TUNE_ALL = 127
+    TUNE_VEC = 64, TUNE_DEEP = 128,
+    TUNE_ALL = 255
 };
 static int tune_flags(const Tune &t) {
     return (t.lds ? TUNE_LDS : 0) | (t.graph ? TUNE_GRAPH : 0) | (t.fused ? TUNE_FUSED : 0) |
            (t.wide ? TUNE_WIDE : 0) | (t.ckpt ? TUNE_CKPT : 0) | (t.narrow ? TUNE_NARROW : 0) |
-           (t.vec ? TUNE_VEC : 0);
+           (t.vec ? TUNE_VEC : 0) | (t.deep ? TUNE_DEEP : 0);
 }
 static Tune tune_from_flags(int bt, int64_t chunk, int flags, int occ) {
     Tune t;
@@ -428,6 +429,7 @@ static Tune tune_from_flags(int bt, int64_t chunk, int flags, int occ) {
     t.ckpt = (flags & TUNE_CKPT) != 0;
     t.narrow = (flags & TUNE_NARROW) != 0;
     t.vec = (flags & TUNE_VEC) != 0;
+    t.deep = (flags & TUNE_DEEP) != 0;
     return t;
 }
 
@@ -571,7 +573,7 @@ inline int64_t pad_rows(int64_t nb) { return (nb + 31) / 32 * 32; }
 // Launch one level with an LDS kernel; returns false (launching nothing) when the level's pair
 // rows do not fit in one workgroup's LDS.  TAG only separates the instantiation used by the
 // gradient's forward pass from the plain forward in profiles; the code is the same.
-template <int DT, typename W, int WPB, int TAG, int KIND = KIND_WIDE>
+template <int DT, typename W, int WPB, int TAG, int KIND = KIND_WIDE, int UB = 4>
 static bool launch_level_lds_w(const DevCaps &caps, const Tune &tune, const DevPlan &pl, int L,
                                const void *P, int64_t ldp, const void *Hc, int64_t ldh, void *Ho,
                                int64_t ldo, int64_t nb, hipStream_t st) {
@@ -582,7 +584,7 @@ static bool launch_level_lds_w(const DevCaps &caps, const Tune &tune, const DevP
     level_pairs(lay, L, &qlo, &nq);
     const size_t lds = (size_t)nq * TW * sizeof(typename E::P);
     if (lds > caps.max_lds_block) return false;
-    auto kern = KIND == KIND_VEC      ? k_level_fwd_lds_vec<DT, W, WPB, TAG>
+    auto kern = KIND == KIND_VEC      ? k_level_fwd_lds_vec<DT, W, WPB, TAG, UB>
                 : KIND == KIND_NARROW ? k_level_fwd_lds_narrow<DT, W, WPB, TAG>
                                       : k_level_fwd_lds<DT, W, WPB, TAG>;
     static bool attr_set = false;
@@ -622,6 +624,11 @@ static bool launch_level_lds(const DevCaps &caps, const Tune &tune, const DevPla
                              const void *P, int64_t ldp, const void *Hc, int64_t ldh, void *Ho,
                              int64_t ldo, int64_t nb, hipStream_t st) {
     if (tune.vec && vec_rows_ok<DT>(Hc, ldh)) {
+        if (tune.deep) {
+            if (tune.wide)
+                return launch_level_lds_w<DT, W, 16, TAG, KIND_VEC, 6>(caps, tune, pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st);
+            return launch_level_lds_w<DT, W, kLdsWaves, TAG, KIND_VEC, 6>(caps, tune, pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st);
+        }
         if (tune.wide)
             return launch_level_lds_w<DT, W, 16, TAG, KIND_VEC>(caps, tune, pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st);
         return launch_level_lds_w<DT, W, kLdsWaves, TAG, KIND_VEC>(caps, tune, pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st);
@@ -1926,6 +1933,11 @@ int eggman_autotune(uint64_t plan, int dtype, int64_t B, int reps, char *report,
                 t.chunk = c;
                 t.occ = occ;
                 t.vec = true;
+                cands.push_back(t);
+                t.wide = true;
+                cands.push_back(t);
+                t.wide = false;
+                t.deep = true;
                 cands.push_back(t);
                 t.wide = true;
                 cands.push_back(t);

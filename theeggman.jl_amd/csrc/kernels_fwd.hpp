@@ -382,7 +382,7 @@ __device__ inline void level_lds_narrow_body(const LevelArgs<DT, W> &a, int64_t 
 //
 // The level rows must be 16-byte aligned with TW | ldh, and a row's padding past the batch must
 // be readable: lanes past the batch read it and store nothing.
-template <int DT, typename W, int WPB, bool A32>
+template <int DT, typename W, int WPB, bool A32, int UB>
 __device__ inline void level_lds_vec_body_a(const LevelArgs<DT, W> &a, int64_t j,
                                             typename Elt<DT>::P *lds) {
     using E = Elt<DT>;
@@ -448,7 +448,7 @@ __device__ inline void level_lds_vec_body_a(const LevelArgs<DT, W> &a, int64_t j
     };
 
     const int64_t step = bstride * WPB * SPW;
-    constexpr int U = 4;
+    constexpr int U = UB;   // child loads of a state issued together (4, or 6 for Tune::deep)
     // a state's words are read once, lane li holding words li, li + 8, ..., and handed to the
     // group's lanes by a shuffle within the 8-lane group
     constexpr int NW = sizeof(W) == 4 ? 4 : 8;   // k <= 31 (Int32 layout) or 63 (Int64)
@@ -498,18 +498,20 @@ __device__ inline void level_lds_vec_body_a(const LevelArgs<DT, W> &a, int64_t j
     }
 }
 
-template <int DT, typename W, int WPB>
+template <int DT, typename W, int WPB, int UB = 4>
 __device__ inline void level_lds_vec_body(const LevelArgs<DT, W> &a, int64_t j,
                                           typename Elt<DT>::P *lds) {
-    if (a.addr32) level_lds_vec_body_a<DT, W, WPB, true>(a, j, lds);
-    else level_lds_vec_body_a<DT, W, WPB, false>(a, j, lds);
+    if (a.addr32) level_lds_vec_body_a<DT, W, WPB, true, UB>(a, j, lds);
+    else level_lds_vec_body_a<DT, W, WPB, false, UB>(a, j, lds);
 }
 
-template <int DT, typename W, int WPB, int TAG>
+// UB: transitions per batch of child loads, 4 or 6 (the autotuner picks; 6 pays with passes of
+// 64 instances at N=28)
+template <int DT, typename W, int WPB, int TAG, int UB = 4>
 __global__ __launch_bounds__(WPB * 64) void k_level_fwd_lds_vec(LevelArgs<DT, W> a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    level_lds_vec_body<DT, W, WPB>(a, (int64_t)blockIdx.x,
-                                   reinterpret_cast<typename Elt<DT>::P *>(lds_raw));
+    level_lds_vec_body<DT, W, WPB, UB>(a, (int64_t)blockIdx.x,
+                                       reinterpret_cast<typename Elt<DT>::P *>(lds_raw));
 }
 
 template <int DT, typename W, int WPB, int TAG>
