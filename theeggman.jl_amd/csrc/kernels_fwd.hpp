@@ -220,8 +220,7 @@ __device__ inline void level_lds_body(const LevelArgs<DT, W> &a, int64_t j,
 
     const int64_t step = bstride * WPB;
     // child loads issued together before they are summed in order; wider batches (8, 12, 16)
-    // measured slower on MI355X at K = 24..32: the level is bound by L2/fabric throughput, not
-    // by the loads one wave keeps in flight
+    // measured slower on MI355X at K = 24..32 when the tail still went one transition at a time
     constexpr int U = 4;
     for (int64_t s0 = s_begin + bi * WPB; s0 < s_end; s0 += step) {
         const int64_t s = wave_uniform(s0 + wave);

@@ -81,7 +81,7 @@ __global__ __launch_bounds__(WPB * 64) void k_rev_g(
         const int64_t e0 = coff[c], e1 = coff[c + 1];
         typename E::H acc = E::zero();
         if (live) {
-            constexpr int U = 4;   // gathers issued together before they are summed in order
+            constexpr int U = 8;   // gathers issued together before they are summed in order
             // N entries from e: their N gathers issued together, then summed in list order
             auto batch = [&](int64_t e, auto n) {
                 constexpr int N = decltype(n)::value;
