@@ -111,7 +111,7 @@ __global__ __launch_bounds__(kPlainBlock) void k_level_fwd_plain(
          s += (int64_t)gridDim.x * kPlainBlock) {
         const W *tw = trans_lvl + s * (int64_t)k;
         HT acc = E::zero();
-        constexpr int U = 8;   // child loads issued together before they are summed in order
+        constexpr int U = 16;   // child loads issued together before they are summed in order
         auto batch = [&](int i, auto n) {
             constexpr int N = decltype(n)::value;
             W t[N];
