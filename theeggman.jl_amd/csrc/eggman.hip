@@ -483,6 +483,10 @@ static Tune pick_tune(const DevPlan &pl, int dtype, int64_t B) {
                    std::get<1>(i->first) == pl.lay.K && std::get<2>(i->first) == pl.lay.index_bytes &&
                    std::get<3>(i->first) == dtype;
         };
+        // a one-instance entry (one lane per state) says nothing about a batch: below 64
+        // instances without an entry of their own, the defaults (as many lanes per state as
+        // instances) serve it
+        if (B > 1 && B < 64) return default_tune(pl.lay, dtype, B);
         if (hi != g_tune.begin() && same(std::prev(hi))) it = std::prev(hi);
         else if (same(hi)) it = hi;
         else return default_tune(pl.lay, dtype, B);
