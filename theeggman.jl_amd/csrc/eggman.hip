@@ -647,22 +647,25 @@ static bool launch_level_lds(const DevCaps &caps, const Tune &tune, const DevPla
     return launch_level_lds_w<DT, W, kLdsWaves, TAG>(caps, tune, pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st);
 }
 
-// One instance: k_level_fwd_plain, its pair vector in LDS (at most 32 KB, K <= 64 complex).
-// One N=28 FP64 matrix: 64.0 -> 61.5 us; one complex N=40 matrix: 12.2 -> 11.1 ms.
-template <int DT, typename W>
+// One or two instances (BT lanes per state): k_level_fwd_plain with the pair values in LDS when
+// they fit in 64 KB; false (nothing launched) otherwise.  One N=28 FP64 matrix: 64.0 -> 61.5 us;
+// one complex N=40 matrix: 12.2 -> 11.1 ms (with batches of 8).  From 4 instances on, the
+// generic kernel's coalesced pair reads beat LDS (N=28, B=8: 103 against 121 us).
+template <int DT, typename W, int BT>
 static bool launch_level_plain(const DevPlan &pl, int L, const void *P, int64_t ldp, const void *Hc,
-                               int64_t ldh, void *Ho, int64_t ldo, hipStream_t st) {
+                               int64_t ldh, void *Ho, int64_t ldo, int64_t nb, hipStream_t st) {
     using E = Elt<DT>;
     const PlanLayout &lay = pl.lay;
     const int64_t nst = lay.nst[L - 1];
     const int k = lay.size[L - 1] - 1;
-    const size_t lds = (size_t)lay.npairs * sizeof(typename E::P);
-    if (lds > 65536 || k < 1) return false;
-    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (nst + kPlainBlock - 1) / kPlainBlock));
+    const size_t lds = (size_t)lay.npairs * BT * sizeof(typename E::P);
+    if (lds > 65536 || k < 1 || nb > BT) return false;
+    const int64_t per_block = kPlainBlock / BT;
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (nst + per_block - 1) / per_block));
     const W *tl = (const W *)pl.d_trans + lay.tstart[L - 1];
-    k_level_fwd_plain<DT, W><<<dim3((unsigned)blocks), dim3(kPlainBlock), lds, st>>>(
+    k_level_fwd_plain<DT, W, BT><<<dim3((unsigned)blocks), dim3(kPlainBlock), lds, st>>>(
         tl, nst, k, (const typename E::P *)P, ldp, (int)lay.npairs, (const typename E::H *)Hc,
-        lay.levels[L - 2], ldh, (typename E::H *)Ho, ldo);
+        lay.levels[L - 2], ldh, (typename E::H *)Ho, ldo, nb);
     return true;
 }
 
@@ -672,10 +675,13 @@ static void launch_level_bt(int bt, const DevPlan &pl, int L, const void *P, int
                             hipStream_t st) {
     switch (bt) {
     case 1:
-        if (nb == 1 && launch_level_plain<DT, W>(pl, L, P, ldp, Hc, ldh, Ho, ldo, st)) break;
+        if (nb == 1 && launch_level_plain<DT, W, 1>(pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st)) break;
         launch_level<DT, W, 1>(pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st);
         break;
-    case 2: launch_level<DT, W, 2>(pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st); break;
+    case 2:
+        if (!launch_level_plain<DT, W, 2>(pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st))
+            launch_level<DT, W, 2>(pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st);
+        break;
     case 4: launch_level<DT, W, 4>(pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st); break;
     case 8: launch_level<DT, W, 8>(pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st); break;
     case 16: launch_level<DT, W, 16>(pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st); break;

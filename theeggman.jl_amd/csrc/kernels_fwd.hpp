@@ -84,31 +84,38 @@ __global__ __launch_bounds__(kBlock) void k_level_fwd(
     }
 }
 
-// One instance, one lane per state, with the instance's pair vector staged in LDS: a lane's pair
-// reads are scattered over the vector, and from LDS they leave the vector memory path to the
-// plan words and the child values.  Each state is summed in plan order, so the values are the
-// other kernels' bits.  P (pair rows ldp apart), Hc and Ho (rows ldh and ldo apart) point at the
-// instance's column.  (Staging the block's plan words in LDS as well, to coalesce their reads,
-// cost more in barriers than it saved.)
+// One or two instances (BT = 1 or 2 lanes per state, one instance each), with the instances' pair
+// values staged in LDS: a lane's pair reads are scattered over the pair rows, and from LDS they
+// leave the vector memory path to the plan words and the child values.  Each lane sums its
+// (state, instance) in plan order, so the values are the other kernels' bits.  P (pair rows ldp
+// apart), Hc and Ho (rows ldh and ldo apart) point at the first instance.  (Staging the block's
+// plan words in LDS as well, to coalesce their reads, cost more in barriers than it saved.)
 constexpr int kPlainBlock = 256;
 
-template <int DT, typename W>
+template <int DT, typename W, int BT>
 __global__ __launch_bounds__(kPlainBlock) void k_level_fwd_plain(
     const W *__restrict__ trans_lvl, int64_t nst, int k,
     const typename Elt<DT>::P *__restrict__ P, int64_t ldp, int npairs,
     const typename Elt<DT>::H *__restrict__ Hc, int64_t cfirst, int64_t ldh,
-    typename Elt<DT>::H *__restrict__ Ho, int64_t ldo) {
+    typename Elt<DT>::H *__restrict__ Ho, int64_t ldo, int64_t nb) {
     using E = Elt<DT>;
     using PT = typename E::P;
     using HT = typename E::H;
     constexpr int bits = Packing<W>::bits;
     constexpr W mask = (W)((W(1) << bits) - 1);
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    PT *Ps = reinterpret_cast<PT *>(lds_raw);
-    for (int q = (int)threadIdx.x; q < npairs; q += kPlainBlock) Ps[q] = P[(int64_t)q * ldp];
+    PT *Ps = reinterpret_cast<PT *>(lds_raw);   // pair q of instance b at Ps[q * BT + b]
+    for (int e = (int)threadIdx.x; e < npairs * BT; e += kPlainBlock) {
+        const int q = e / BT, bb = e % BT;
+        Ps[e] = bb < nb ? P[(int64_t)q * ldp + bb] : PT{};
+    }
     __syncthreads();
-    for (int64_t s = (int64_t)blockIdx.x * kPlainBlock + threadIdx.x; s < nst;
-         s += (int64_t)gridDim.x * kPlainBlock) {
+    const int b = (int)(threadIdx.x % BT);
+    const PT *Pl = Ps + b - BT;                    // pair row q (1-based) of this lane: Pl[q * BT]
+    const int64_t per_block = kPlainBlock / BT;
+    for (int64_t s = (int64_t)blockIdx.x * per_block + threadIdx.x / BT; s < nst;
+         s += (int64_t)gridDim.x * per_block) {
+        if (b >= nb) continue;
         const W *tw = trans_lvl + s * (int64_t)k;
         HT acc = E::zero();
         constexpr int U = 16;   // child loads issued together before they are summed in order
@@ -119,14 +126,14 @@ __global__ __launch_bounds__(kPlainBlock) void k_level_fwd_plain(
 #pragma unroll
             for (int u = 0; u < N; u++) t[u] = tw[i + u];
 #pragma unroll
-            for (int u = 0; u < N; u++) h[u] = Hc[((int64_t)(t[u] >> bits) - cfirst) * ldh];
+            for (int u = 0; u < N; u++) h[u] = Hc[((int64_t)(t[u] >> bits) - cfirst) * ldh + b];
 #pragma unroll
-            for (int u = 0; u < N; u++) acc = E::muladd(acc, Ps[(int)(t[u] & mask) - 1], h[u]);
+            for (int u = 0; u < N; u++) acc = E::muladd(acc, Pl[(int)(t[u] & mask) * BT], h[u]);
         };
         int i = 0;
         for (; i + U <= k; i += U) batch(i, std::integral_constant<int, U>{});
         batch_tail<U>(k - i, [&](auto n) { batch(i, n); });
-        Ho[s * ldo] = acc;
+        Ho[s * ldo + b] = acc;
     }
 }
 
