@@ -449,6 +449,7 @@ static Tune default_tune(const PlanLayout &lay, int, int64_t B) {
     t.bt = pow2_at_least(B);
     t.chunk = B;
     t.fused = true;
+    t.lds = B >= 64;
     if (B >= 64) {
         t.lds = true;
         t.vec = true;
@@ -1476,7 +1477,9 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
             const char *Ps = Pc + (size_t)s0 * pb;
             const void *Hc = level_rows(L - 1) + s0;
             void *Ho = L == top ? (void *)(outc + s0) : (void *)(level_rows(L) + s0);
-            if (!(tune.lds && launch_level_lds<DT, W, 1>(ds.caps, tune, pl, L, Ps, ldp, Hc, ldh, Ho, ldh, ns, st)))
+            // the LDS kernels work on tiles of 16 or 64 instances: fewer take the small-batch ones
+            if (!(tune.lds && ns >= 64 &&
+                  launch_level_lds<DT, W, 1>(ds.caps, tune, pl, L, Ps, ldp, Hc, ldh, Ho, ldh, ns, st)))
                 launch_level_bt<DT, W>(pow2_at_least(ns), pl, L, Ps, ldp, Hc, ldh, Ho, ldh, ns, st);
         };
         auto forward_range = [&](int L0, int L1) {
