@@ -231,7 +231,8 @@ def test_device_resident_entry_point():
 
 
 @pytest.mark.parametrize("T,N,B", [(np.float64, 26, 100), (np.float64, 18, 70), (np.float32, 30, 64),
-                                   (np.complex64, 26, 80), (np.float64, 18, 1)])
+                                   (np.complex64, 26, 80), (np.float64, 18, 1), (np.float64, 28, 8),
+                                   (np.complex128, 24, 33)])
 def test_untuned_defaults_bit_exact(T, N, B):
     # degrees and element types with no tuning entry use the default launch choices (the fused
     # walk, or the 16-byte-load kernel in passes); the values are the oracle's bits
