@@ -36,6 +36,44 @@ def alg_bytes_fwd(K, B, tbytes, ibytes, ns, nt):
     return (nt + ns + 1) * ibytes + B * (nt + ns + npairs) * tbytes
 
 
+def walk_split(nst, B, chunk, small=4096):
+    """How the library walks B instances in passes of `chunk`: (passes, lb, lt) where levels
+    2..lb and lt..top (at most `small` states each) run once for the whole batch when there is
+    more than one pass, and levels lb+1..lt-1 once per pass.  nst[L-1] = states of level L."""
+    top = len(nst)
+    passes = -(-B // chunk)
+    lb, lt = 1, top + 1
+    if passes > 1:
+        while lb + 1 < top and nst[lb] <= small:
+            lb += 1
+        while lt - 1 > lb + 1 and nst[lt - 2] <= small:
+            lt -= 1
+    return passes, lb, lt
+
+
+def compulsory_bytes(nst, K, B, chunk, tbytes, ibytes, small=4096):
+    """HBM bytes one forward call cannot avoid: every state value written once and read once as
+    a child by the level above (the top level's one value per instance is the output), the plan
+    words of a level once per walk of that level (once per pass, or once for the whole batch for
+    the small end levels), and the pair values once."""
+    reads, writes = compulsory_split(nst, K, B, chunk, tbytes, ibytes, small)
+    return reads + writes
+
+
+def compulsory_split(nst, K, B, chunk, tbytes, ibytes, small=4096):
+    """(reads, writes) of compulsory_bytes."""
+    passes, lb, lt = walk_split(nst, B, chunk, small)
+    top = len(nst)
+    states = sum(int(n) for n in nst)
+    writes = B * (states - 1) * tbytes                     # levels 2..top
+    reads = B * (states - int(nst[-1])) * tbytes           # levels 1..top-1, each once as a child
+    for L in range(2, top + 1):
+        k = 2 * (L - 1) - 1                       # out-degree of level L (subsets of size 2(L-1))
+        walks = passes if lb < L < lt else 1
+        reads += walks * int(nst[L - 1]) * k * ibytes
+    return reads + B * (K * (K - 1) // 2) * tbytes, writes
+
+
 def level_launches(nst, B, chunk, small=4096):
     """Level launches of one forward call of B instances in passes of `chunk`, as the library
     makes them: every level once per pass, except that when there is more than one pass the
@@ -71,44 +109,117 @@ def time_device(hip, stream, fn, steps, warmup):
     return ev0.elapsed_ms(ev1) / steps, wall * 1e3 / steps
 
 
-def traffic_from_profiles():
-    """HBM bytes per launch of the level kernel from the committed rocprofv3 PMC summary, if
-    present (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_n28_b1024.json")
+TRAFFIC_PROFILE = os.path.join("profiles", "hbm_traffic_n28_b1024.json")
+
+
+def traffic_from_profiles(kernel, chunk):
+    """HBM bytes per launch of the level kernel from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE runs of this workload, scaled by the copy-kernel calibration), with
+    its provenance -- or None when the summary is missing or was taken for another kernel or pass
+    size than the launch timed here."""
+    path = os.path.join(ROOT, TRAFFIC_PROFILE)
     if not os.path.exists(path):
         return None
     try:
         with open(path) as f:
-            return json.load(f).get("bytes_per_launch")
+            prof = json.load(f)
     except (OSError, ValueError):
         return None
+    if prof.get("kernel", "").split("<")[0] != kernel or prof.get("chunk", chunk) != chunk:
+        return None
+    return prof
 
 
-def cpu_baseline(K, budget_s=12.0):
-    """The oracle's C restatement of the reference CPU walk (_batch_cpu! over _haf_dp,
-    src/hafnian.jl:1226-1242, src/dp.jl:408-442) on this host's cores: min over reps after one
-    warmup, plan build excluded."""
+# The reference's published single-call CPU figure for this path: `hafnian(A; check_symmetric =
+# false)` on one ComplexF64 N=28 matrix, min over samples, 32 threads of a Ryzen 9 9950X, Julia
+# 1.12.7 (assets/data/jl-eggman-hafnian-bench.json, key haf.eggman.rpt1.N=28).
+PUBLISHED_CPU_N28_CF64_MS = 3.28
+
+
+def host_cpu():
+    """CPU model and SMT state of this host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    smt = None
+    try:
+        with open("/sys/devices/system/cpu/smt/active") as f:
+            smt = f.read().strip() == "1"
+    except OSError:
+        pass
+    return {"model": model, "smt_active": smt, "logical_cpus": os.cpu_count()}
+
+
+def native_oracle():
+    """Build the oracle's C restatement with -march=native for this host (the CPU baseline's
+    flags: -O3 -march=native -ffp-contract=off) into a temporary directory and load it; fall back
+    to the portable build when no compiler is available."""
+    import subprocess
+    import tempfile
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
     import oracle
+    out = os.path.join(tempfile.mkdtemp(prefix="eggman_oracle_"), "liboracle_native.so")
+    cmd = ["gcc", "-O3", "-march=native", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared",
+           "-o", out, os.path.join(ROOT, "oracle", "dp_oracle.c"), "-lquadmath"]
+    try:
+        subprocess.run(cmd, check=True, capture_output=True, timeout=120)
+    except (OSError, subprocess.SubprocessError):
+        return oracle, "portable build (-march=x86-64-v3): native build failed"
+    oracle.use_library(out)
+    return oracle, "gcc -O3 -march=native -ffp-contract=off -fopenmp"
+
+
+def cpu_baseline(K, budget_s=12.0, plain_budget_s=6.0):
+    """The oracle's C restatement of the reference CPU walk on this host's cores, min over reps
+    after one warmup, plan build excluded (as benchmark/gpu/run_gpu_validation.jl:73-83 times):
+
+    * batched (the reported value): _batch_cpu! over _haf_dp (src/hafnian.jl:1226-1242), B split
+      into nthreads contiguous chunks, each instance serial;
+    * plain: one N=K matrix by _haf_dp with its level-chunk policy (2000 transitions per chunk,
+      parallel from 60000 transitions; src/dp.jl:370-442) on all threads, in Float64 and in
+      ComplexF64 -- the latter beside the reference's published 3.28 ms."""
+    import numpy as np
     from theeggman import synth
+    oracle, build = native_oracle()
     nthreads = min(os.cpu_count() or 1, oracle.max_threads())
     Bs = max(nthreads * 32, 256)
     As = np.stack([synth.sym_normal(K, 3, b, 1 / np.sqrt(K)) for b in range(Bs)])
     P = synth.pair_batch(As)
     oracle.build_plan(K)
-    oracle.haf_dp(P, K, oracle.F64, nthreads)            # warmup
-    best, spent, reps = float("inf"), 0.0, 0
-    while spent < budget_s and reps < 200:
-        t0 = time.perf_counter()
-        oracle.haf_dp(P, K, oracle.F64, nthreads)
-        dt = time.perf_counter() - t0
-        best, spent, reps = min(best, dt), spent + dt, reps + 1
+
+    def best_of(fn, budget, cap):
+        fn()                                               # warmup
+        best, spent, reps = float("inf"), 0.0, 0
+        while spent < budget and reps < cap:
+            t0 = time.perf_counter()
+            fn()
+            dt = time.perf_counter() - t0
+            best, spent, reps = min(best, dt), spent + dt, reps + 1
+        return best, spent, reps
+
+    best, spent, reps = best_of(lambda: oracle.haf_dp(P, K, oracle.F64, nthreads), budget_s, 200)
     _, nt = oracle.dp_counts(K)
+    plain = {}
+    for name, T, dt, A in (("f64", np.float64, oracle.F64, synth.sym_normal(K, 3, 0, 1 / np.sqrt(K))),
+                           ("c64", np.complex128, oracle.C64, synth.sym_complex_normal(K, 2, 0))):
+        P1 = synth.pair_batch(A[None]).astype(T)
+        b1, sp1, r1 = best_of(lambda: oracle.haf_dp(P1, K, dt, nthreads), plain_budget_s / 2, 2000)
+        plain[name] = {"ms_per_hafnian": b1 * 1e3, "transitions_per_s": nt / b1,
+                       "reps": r1, "threads": nthreads}
+    plain["c64"]["published_reference_ms"] = PUBLISHED_CPU_N28_CF64_MS if K == 28 else None
+    if K == 28:
+        plain["c64"]["ratio_to_published"] = plain["c64"]["ms_per_hafnian"] / PUBLISHED_CPU_N28_CF64_MS
     return {"value": Bs * nt / best, "unit": "DP transitions/s", "cores": nthreads,
             "kind": "port", "hafnians_per_s": Bs / best,
             "sample": f"{Bs} real Float64 N={K} matrices per rep, _batch_cpu! policy over "
-                      f"{nthreads} threads, min of {reps} reps ({spent:.1f} s)"}
+                      f"{nthreads} threads, min of {reps} reps ({spent:.1f} s)",
+            "build": build, "host": host_cpu(), "plain": plain}
 
 
 def oracle_checks(P, res, K, nsample=16):
@@ -283,13 +394,17 @@ def main():
     res = out.download(np.float64, B)
 
     value = B * nt / (ms / 1e3)
-    nbytes = alg_bytes_fwd(K, B, 8, ib, ns, nt)
-    chunk = (tuned or {}).get("chunk") or B
-    launches = level_launches(np.diff(np.asarray(plan.levels, dtype=np.int64)), B, min(chunk, B))
+    chunk = min((tuned or {}).get("chunk") or B, B)
+    nst = np.diff(np.asarray(plan.levels, dtype=np.int64))
+    launches = level_launches(nst, B, chunk)
     tl = tuned or {}
     head = ("k_level_fwd_lds_vec" if tl.get("vec") else
             "k_level_fwd_lds_narrow" if tl.get("narrow") else "k_level_fwd_lds") \
         if tl.get("lds", 1) and not tl.get("fused") else "k_level_fwd"
+    # the roofline's algorithmic bytes are the compulsory traffic of the step; the
+    # per-transition model (one 8-byte child read per transition) is reported beside it, labelled
+    nbytes = compulsory_bytes(nst, K, B, chunk, 8, ib)
+    model_bytes = alg_bytes_fwd(K, B, 8, ib, ns, nt)
     achieved = nbytes / (ms / 1e3) / 1e9
     result = {
         "metric": "DP transitions/s (batched N=28 FP64 forward)",
@@ -312,13 +427,30 @@ def main():
         "oracle_spot_check": None,
         "tuned_launch": tuned,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": head, "launches_per_step": launches,
                      "alg_bytes_per_launch": nbytes / launches,
-                     "avg_launch_ms": ms / launches},
+                     "alg_bytes": "compulsory: each state value written once and read once, "
+                                  "plan words once per walk of a level, pair values once",
+                     "avg_launch_ms": ms / launches,
+                     "per_transition_model": {
+                         "bytes_per_step": model_bytes,
+                         "gbs": model_bytes / (ms / 1e3) / 1e9,
+                         "note": "one 8-byte child read per transition; the colex sweep serves "
+                                 "most of them from L2/MALL, so this rate can exceed the HBM peak "
+                                 "and is not a roofline fraction"}},
     }
 
     rl = result["roofline"]
+    prof = traffic_from_profiles(head, chunk)
+    if prof:
+        rl["traffic"] = prof["bytes_per_launch"]
+        rl["traffic_source"] = (f"{TRAFFIC_PROFILE}: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
+                                f"runs of this workload ({prof.get('measured_at', 'see file')}), "
+                                f"FETCH scaled by {prof.get('fetch_scale')}")
+        creads, cwrites = compulsory_split(nst, K, B, chunk, 8, ib)
+        rl["read_amplification"] = prof["fetch_bytes_per_launch"] * launches / creads
+        rl["write_amplification"] = prof["write_bytes_per_launch"] * launches / cwrites
     if args.copy_calibration <= 0:
         # measured device-copy bandwidth on this GPU (reads + writes of a 2 GiB copy), the
         # practical ceiling beside the nominal 8 TB/s (skipped in counter runs, whose one copy

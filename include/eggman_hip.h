@@ -37,6 +37,8 @@ extern "C" {
 #define EGGMAN_EPLAN (-3)        /* plan cannot be represented, or is inconsistent */
 #define EGGMAN_EHIP (-4)         /* a HIP runtime call failed */
 #define EGGMAN_EUNSUPPORTED (-5) /* valid request this build does not handle */
+#define EGGMAN_EOVERFLOW (-6)    /* EGGMAN_I128 request whose exact result may not fit in 127 bits
+                                    (maps to OverflowError) */
 
 /* element types (the T of Matrix{T}) */
 #define EGGMAN_F32 1  /* Float32 */
@@ -44,7 +46,13 @@ extern "C" {
 #define EGGMAN_C32 3  /* ComplexF32, interleaved */
 #define EGGMAN_C64 4  /* ComplexF64, interleaved */
 #define EGGMAN_I128 5 /* exact: P holds int64 entries; results are 128-bit two's-complement
-                         integers (lo, hi uint64 limbs), exact modulo 2^128 */
+                         integers (lo, hi uint64 limbs).  Every entry point refuses
+                         (EGGMAN_EOVERFLOW, before any walk) an instance whose hafnian is not
+                         provably below 2^127 in magnitude: the bound is the smaller of
+                         (K-1)!! * m^(K/2) and sqrt(prod_i max(r_i, 1)), with m = max(1, max |p|)
+                         and r_i = sum_{j != i} |p_ij| (haf^2 <= perm <= prod r_i for |A|); it
+                         also bounds every (K-2)-minor, so gradients are covered.  Device-pointer
+                         entry points check on the device and synchronise the stream for it. */
 
 /* Version string of the library. */
 const char *eggman_version(void);
@@ -225,6 +233,14 @@ int eggman_tune_set(int K, int index_bytes, int dtype, int64_t B, int bt, int64_
                     int flags, int occ);
 int eggman_tune_get(int K, int index_bytes, int dtype, int64_t B, int *bt, int64_t *chunk,
                     int *flags, int *occ);
+
+/*
+ * The range check of the exact walk, host only: log2 of the bound described at EGGMAN_I128 for
+ * each of the B instances of P (int64, [pair][ldp], npairs = K(K-1)/2 rows) into log2_bound
+ * (B doubles; may be NULL).  Returns EGGMAN_EOVERFLOW if an instance's bound reaches 2^127,
+ * which every EGGMAN_I128 entry point refuses.  New: the reference has no exact path.
+ */
+int eggman_exact_bound(int K, const int64_t *P, int64_t ldp, int64_t B, double *log2_bound);
 
 /* ---------------------------------------------------------------- caches */
 

@@ -8,7 +8,8 @@
  * What is restated, with the reference lines each function follows:
  *   oracle_dp_counts      src/dp.jl:192-208   closed-form (nstates, ntrans)
  *   oracle_build_plan     src/dp.jl:223-304   guards, DFS over reachable masks, sort by
- *                                             (popcount, mask), rank map, packed CSR, levels
+ *                                             (popcount, mask), rank map (a hash map, as the
+ *                                             reference's Dict), packed CSR, levels
  *   pair_index            src/unrolled.jl:60  flat upper-triangle index, 1-based
  *   dp_states_*           src/dp.jl:344-363   the inner loop, plan order, no FMA
  *   haf_dp_one_*          src/dp.jl:408-442   level driver with the _dp_level_chunks policy
@@ -112,20 +113,18 @@ static int cmp_popmask(const void *a, const void *b) {
     return x < y ? -1 : (x > y ? 1 : 0);
 }
 
-/* the rank map `number` of dp.jl:272, as a binary search over the sorted masks */
-static int64_t rank_of(const uint64_t *masks, int64_t n, uint64_t m) {
-    int64_t lo = 0, hi = n - 1;
-    while (lo <= hi) {
-        int64_t mid = (lo + hi) >> 1;
-        int c = cmp_popmask(&masks[mid], &m);
-        if (c == 0) return mid + 1;               /* 1-based state number */
-        if (c < 0) lo = mid + 1; else hi = mid - 1;
-    }
-    return -1;
+/* slot of a nonzero mask present in the set (read-only, so threads may share it) */
+static uint64_t maskset_slot(const maskset *s, uint64_t m) {
+    uint64_t h = mix64(m) & (s->cap - 1);
+    while (s->keys[h] != m) h = (h + 1) & (s->cap - 1);
+    return h;
 }
 
 static void st(void *a, int w, int64_t i, int64_t v) {
     if (w == 4) ((int32_t *)a)[i] = (int32_t)v; else ((int64_t *)a)[i] = v;
+}
+static int64_t ld(const void *a, int w, int64_t i) {
+    return w == 4 ? (int64_t)((const int32_t *)a)[i] : ((const int64_t *)a)[i];
 }
 
 /*
@@ -182,27 +181,43 @@ int oracle_build_plan(int K, int index_bytes, void *starts, void *trans, void *l
     if (set.has_zero) masks[n++] = 0;
     for (uint64_t h = 0; h < set.cap; h++)
         if (set.keys[h]) masks[n++] = set.keys[h];
-    free(set.keys);
     qsort(masks, (size_t)nstates, sizeof(uint64_t), cmp_popmask);
 
-    int64_t t = 0;
+    /* the rank map `number` of dp.jl:272 (a Dict{M,I} there): the set's own hash table, with a
+       1-based state number per slot; the empty mask is state 1 (it sorts first) */
+    int64_t *number = (int64_t *)malloc(sizeof(int64_t) * (size_t)set.cap);
+    if (!number) { free(set.keys); free(masks); return OR_ENOMEM; }
+#pragma omp parallel for schedule(static)
+    for (int64_t s = 0; s < nstates; s++)
+        if (masks[s] != 0) number[maskset_slot(&set, masks[s])] = s + 1;
+
+    /* starts: a state S has |S| - 1 transitions (none for the empty set) */
     st(starts, index_bytes, 0, 1);
     for (int64_t s = 0; s < nstates; s++) {
-        uint64_t m = masks[s];
-        if (m != 0) {
-            int i = __builtin_ctzll(m) + 1;
-            uint64_t rest = m & ~((uint64_t)1 << (i - 1));
-            uint64_t r = rest;
-            while (r != 0) {
-                int j = __builtin_ctzll(r) + 1;
-                r &= ~((uint64_t)1 << (j - 1));
-                int64_t c = rank_of(masks, nstates, rest & ~((uint64_t)1 << (j - 1)));
-                st(trans, index_bytes, t, (c << bits) | pair_index(i, j, K));
-                t++;
-            }
-        }
-        st(starts, index_bytes, s + 1, t + 1);
+        int pc = __builtin_popcountll(masks[s]);
+        st(starts, index_bytes, s + 1, ld(starts, index_bytes, s) + (pc > 0 ? pc - 1 : 0));
     }
+    /* transitions in ascending j (dp.jl:286-292); every state writes its own slice, so the
+       states are emitted in parallel with the same result */
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int64_t s = 0; s < nstates; s++) {
+        uint64_t m = masks[s];
+        if (m == 0) continue;
+        int64_t t = ld(starts, index_bytes, s) - 1;
+        int i = __builtin_ctzll(m) + 1;
+        uint64_t rest = m & ~((uint64_t)1 << (i - 1));
+        uint64_t r = rest;
+        while (r != 0) {
+            int j = __builtin_ctzll(r) + 1;
+            r &= ~((uint64_t)1 << (j - 1));
+            uint64_t sub = rest & ~((uint64_t)1 << (j - 1));
+            int64_t c = sub == 0 ? 1 : number[maskset_slot(&set, sub)];
+            st(trans, index_bytes, t, (c << bits) | pair_index(i, j, K));
+            t++;
+        }
+    }
+    free(number);
+    free(set.keys);
     int64_t nl = 0;
     st(levels, index_bytes, nl++, 1);
     for (int64_t s = 1; s < nstates; s++)

@@ -35,6 +35,15 @@ DP_HARD_MAX = 64     # src/dp.jl:124
 _lib = None
 
 
+def use_library(path: str) -> None:
+    """Load the oracle from another build of dp_oracle.c (the CPU baseline times a
+    ``-march=native`` build made on the machine it runs on)."""
+    global _lib, LIB_PATH
+    LIB_PATH = path
+    _lib = None
+    _PLANS.clear()
+
+
 def lib():
     """Load liboracle.so (built by ``make -C oracle``)."""
     global _lib

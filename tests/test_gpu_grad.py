@@ -156,7 +156,34 @@ def test_rrule_pullback():
     _, G = te.hafnian_grad(A)
     assert y == te.hafnian(A)
     none, dA = pb(2.5)
-    assert none is None and np.array_equal(dA, 2.5 * G)
+    # the cotangent of a function of a symmetric matrix: each half of a pair carries G_ij / 2
+    assert none is None and np.array_equal(dA, 2.5 * G / 2)
+    # a symmetric tangent: <dA_bar, E> = sum_{i<j} G_ij E_ij, the directional derivative
+    E = synth.sym_uniform(12, 33, 1)
+    assert np.isclose(np.sum(dA / 2.5 * E), np.sum(np.triu(G, 1) * np.triu(E, 1)), rtol=1e-13)
+
+
+@pytest.mark.parametrize("T", [np.float64, np.complex128])
+def test_rrule_through_symmetrising_chain(T):
+    # f(X) = hafnian(X + X^T) for a non-symmetric X: the pullback of X -> X + X^T is
+    # dX = dA + dA^T, and dX[i, j] must be the derivative of f in X[i, j] -- central differences
+    # (exact up to rounding: f is affine in each entry), conjugated for complex X (ChainRules'
+    # dy * conj(f') convention)
+    N, h = 10, 1e-4
+    r = np.random.default_rng(34)
+    X = r.uniform(-1, 1, (N, N)).astype(T)
+    if np.dtype(T).kind == "c":
+        X = X + 1j * r.uniform(-1, 1, (N, N))
+    _, pb = te.rrule(te.hafnian, X + X.T)
+    _, dA = pb(1.0)
+    dX = dA + dA.T
+    for (i, j) in [(0, 1), (3, 7), (8, 2), (9, 0)]:
+        E = np.zeros((N, N), T)
+        E[i, j] = h
+        fp = te.hafnian((X + E) + (X + E).T)
+        fm = te.hafnian((X - E) + (X - E).T)
+        fd = (fp - fm) / (2 * h)
+        assert abs(np.conj(dX[i, j]) - fd) <= 1e-8 * max(1.0, abs(fd)), (i, j)
 
 
 @pytest.mark.parametrize("N,B", [(28, 4), (32, 1)])

@@ -1,12 +1,15 @@
 """Degrees 36 and 40 on the Int64 plan layout, generated in HBM (21 GiB at K=40).
 
-The oracle cannot walk these degrees in test time, so parity rests on identities that fix the
-exact answer and exercise the whole plan: closed forms of graph families, the permanent of a
+The oracle cannot walk these degrees in test time, so parity rests on committed exact counts of
+eight G(40, 1/2) graphs (the oracle's 128-bit walk, run once by tests/golden/make_n40_exact.py)
+and on identities that fix the exact answer and exercise the whole plan: closed forms of graph families, the permanent of a
 bipartite block, the product rule for a block-diagonal matrix whose blocks the oracle evaluates
 at half the degree, all hidden behind a random relabeling of the vertices -- and invariance under
 relabeling itself, bit-exact on the 128-bit integer path.
 """
+import json
 import math
+import os
 
 import numpy as np
 import pytest
@@ -17,6 +20,10 @@ from theeggman import _lib, synth
 
 pytestmark = pytest.mark.gpu
 GEN = te.HIPBackend(plan_source="generate")
+# exact counts of eight of the 64 graphs, from the C oracle's 128-bit walk over its own host-built
+# K=40 plan (tests/golden/make_n40_exact.py)
+GOLD40 = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                     "n40_exact.json")))
 
 
 def note(msg):
@@ -87,6 +94,8 @@ def test_n40_64_random_graphs_exact_and_relabel_invariant():
     As = [synth.gnp_adjacency(40, 52, b) for b in range(64)]
     counts = exact(As)
     assert all(isinstance(c, int) and 0 <= c < math.prod(range(39, 0, -2)) for c in counts)
+    for b, c in GOLD40["counts"].items():
+        assert counts[int(b)] == c, b
     again = exact([relabel(A, 100 + b) for b, A in enumerate(As)])
     assert again == counts
     note(f"  counts[0:3] = {counts[:3]}")

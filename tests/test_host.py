@@ -3,6 +3,7 @@ builder and validator against the oracle, and the router's validation and refusa
 raise before any device call)."""
 import ctypes
 import json
+import math
 import os
 import re
 
@@ -228,3 +229,58 @@ def test_gradient_checkpoints_save_memory_at_large_degree():
         kr, sr, _ = te.grad_levels(K, checkpointed=True)
         full, _, _ = te.grad_levels(K, checkpointed=False)
         assert kr + sr < 0.75 * full, K
+
+
+# ------------------------------------------------------------- range check of the exact walk
+
+def _exact_bound(As):
+    As = np.asarray(As, dtype=np.int64)
+    K = As.shape[-1]
+    P = np.ascontiguousarray(synth.pair_batch(As).astype(np.int64))
+    B = P.shape[1]
+    lb = np.empty(B)
+    rc = _lib.lib().eggman_exact_bound(K, P.ctypes.data, B, B, lb.ctypes.data)
+    return rc, lb
+
+
+def test_exact_bound_is_an_upper_bound():
+    # log2 |haf| <= the bound, with haf from the oracle's 128-bit walk (exact in this range),
+    # for random signed matrices, sparse ones and the all-equal matrices that meet (N-1)!! m^(N/2)
+    rng = np.random.default_rng(7)
+    for N in (4, 8, 12, 16):
+        for m in (1, 3, 100, 1 << 20):
+            A = np.triu(rng.integers(-m, m + 1, (N, N)), 1)
+            A = A + A.T
+            if m == 3:
+                A[rng.random((N, N)) < 0.5] = 0
+                A = np.triu(A, 1) + np.triu(A, 1).T
+            rc, lb = _exact_bound([A, np.full((N, N), m)])
+            assert (rc == _lib.OK) == (lb.max() < 127 - 1e-6)
+            if rc != _lib.OK:
+                continue       # beyond the 128-bit walk's range, so no exact value to compare
+            for M, b in ((A, lb[0]), (np.full((N, N), m), lb[1])):
+                h = o.haf_dp(o.gather_pairs(M).astype(np.int64).reshape(-1, 1), N, o.I128)[0]
+                assert h == 0 or abs(h).bit_length() - 1 <= b + 1e-9, (N, m)
+                assert h == 0 or math.log2(abs(h)) <= b + 1e-9, (N, m)
+    # the all-ones matrix meets the first bound: log2 (N-1)!!
+    _, lb = _exact_bound([np.ones((16, 16), np.int64)])
+    assert abs(lb[0] - np.log2(float(np.prod(np.arange(15, 0, -2))))) < 1e-9
+
+
+def test_exact_bound_refuses_what_128_bits_cannot_hold():
+    # every off-diagonal entry 2^62 at N=6: haf = 15 * 2^186, which the 128-bit walk wraps to 0
+    A = np.full((6, 6), 1 << 62, np.int64)
+    assert o.haf_dp(o.gather_pairs(A).reshape(-1, 1), 6, o.I128)[0] == 0
+    rc, lb = _exact_bound([A])
+    assert rc == _lib.EOVERFLOW and lb[0] > 186
+    assert "127" in _lib.last_error()
+    # K_40 with every edge 5: 39!! * 5^20 ~ 2^124.5 fits; with 6 (~2^129.8) it may not
+    rc5, lb5 = _exact_bound([np.full((40, 40), 5, np.int64)])
+    rc6, _ = _exact_bound([np.full((40, 40), 6, np.int64)])
+    assert rc5 == _lib.OK and 124 < lb5[0] < 125 and rc6 == _lib.EOVERFLOW
+    # G(40, 1/2) adjacency matrices are far inside the range (row-sum bound ~ 2^88)
+    rc, lb = _exact_bound([synth.gnp_adjacency(40, 52, b) for b in range(4)])
+    assert rc == _lib.OK and (lb < 90).all()
+    # extreme entries do not break the bound's arithmetic
+    rc, _ = _exact_bound([np.full((4, 4), np.iinfo(np.int64).min, np.int64)])
+    assert rc == _lib.EOVERFLOW

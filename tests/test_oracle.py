@@ -180,6 +180,80 @@ def test_i128_wraps_exactly_past_64_bits():
     assert haf_int(B) == want
 
 
+# ---------------------------------------------------- single precision and the complex quad walk
+
+def small_int_sym(seed, N, lo, hi):
+    A = np.random.default_rng(seed).integers(lo, hi + 1, (N, N))
+    A = np.triu(A, 1)
+    return A + A.T
+
+
+@pytest.mark.parametrize("N,m", [(10, 3), (12, 2), (14, 1)])
+def test_f32_walk_exact_on_small_integers(N, m):
+    # entries in [-m, m]: every state value, product and partial sum is an integer of magnitude
+    # at most (N-1)!! * m^(N/2) < 2^24, so float32 arithmetic is exact and the Float32 walk must
+    # equal the 128-bit integer walk (a wrong pair, child or level changes the integer)
+    assert math.prod(range(N - 1, 0, -2)) * m ** (N // 2) < 2 ** 24
+    for seed in range(3):
+        A = small_int_sym(100 * N + seed, N, -m, m)
+        v = haf_f64(A.astype(np.float32), o.F32)
+        assert v.dtype == np.float32
+        assert int(v) == v and int(v) == haf_int(A)
+        # the Int64 layout walks the same sums
+        assert haf_f64(A.astype(np.float32), o.F32, ib=8) == v
+
+
+@pytest.mark.parametrize("N", [10, 12, 14])
+def test_c32_walk_exact_on_gaussian_integers(N):
+    # entries a + bi with a, b in {-1, 0, 1}: |state| <= (N-1)!! * sqrt(2)^(N/2) < 2^24, so the
+    # ComplexF32 walk is exact and must equal the same walk over Python complex numbers, whose
+    # double arithmetic is exact for these integers (a slip in (ac - bd, ad + bc) fails it)
+    assert math.prod(range(N - 1, 0, -2)) * 2 ** (N / 4) < 2 ** 24
+    for seed in range(3):
+        A = small_int_sym(200 * N + seed, N, -1, 1) + 1j * small_int_sym(300 * N + seed, N, -1, 1)
+        v = haf_f64(A.astype(np.complex64), o.C32)
+        assert v.dtype == np.complex64
+        want = o.py_haf_dp([complex(x) for x in o.gather_pairs(A)], o.py_build_plan(N))
+        assert complex(v) == want
+        assert want == complex(haf_int(A.real.astype(np.int64)), 0) or A.imag.any()
+
+
+class _CFrac:
+    """Exact complex rationals, for the complex walk over Fractions."""
+
+    def __init__(self, re, im=Fraction(0)):
+        self.re, self.im = Fraction(re), Fraction(im)
+
+    def __add__(self, o_):
+        return _CFrac(self.re + o_.re, self.im + o_.im)
+
+    def __mul__(self, o_):
+        return _CFrac(self.re * o_.re - self.im * o_.im, self.re * o_.im + self.im * o_.re)
+
+
+@pytest.mark.parametrize("N", [12, 16])
+def test_cq_walk_against_exact_complex_rationals(N):
+    # complex dyadic inputs: the complex __float128 walk is within its 113-bit rounding of the
+    # exact value of the same walk over complex Fractions, component by component
+    r = np.random.default_rng(400 + N)
+    A = (np.round(r.uniform(-1, 1, (N, N)) * 2**12) + 1j * np.round(r.uniform(-1, 1, (N, N)) * 2**12)) / 2**12
+    A = np.triu(A, 1)
+    A = A + A.T
+    P = o.gather_pairs(A)
+    exact = o.py_haf_dp([_CFrac(x.real, x.imag) for x in P], o.py_build_plan(N),
+                        one=_CFrac(1), zero=_CFrac(0))
+    hi, lo = o.haf_dp(P.reshape(-1, 1), N, o.CQ)[0]
+    scale = abs(exact.re) + abs(exact.im)
+    got_re = Fraction(float(hi.real)) + Fraction(float(lo.real))
+    got_im = Fraction(float(hi.imag)) + Fraction(float(lo.imag))
+    assert abs(got_re - exact.re) <= Fraction(1, 10**28) * scale
+    assert abs(got_im - exact.im) <= Fraction(1, 10**28) * scale
+    # and the ComplexF64 walk within double rounding of it
+    v = haf_f64(A, o.C64)
+    assert abs(Fraction(float(v.real)) - exact.re) <= Fraction(1, 10**12) * scale
+    assert abs(Fraction(float(v.imag)) - exact.im) <= Fraction(1, 10**12) * scale
+
+
 def double_factorial(n):
     return math.prod(range(n, 0, -2))
 

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -113,6 +114,7 @@ struct DevState {
     DevBuf G;                        // adjoint values (ping-pong levels)
     DevBuf part;                     // per-segment partial gradients
     DevBuf A;                        // staged matrix (repeated-pattern batches)
+    DevBuf bnd;                      // per-instance range bounds of exact device-pointer calls
     // host entry points: a two-slot pipeline of pinned host buffers and device buffers, with an
     // upload and a download stream beside the compute stream (run_pipe)
     hipStream_t up = nullptr, down = nullptr;
@@ -120,8 +122,8 @@ struct DevState {
     PinBuf pin_in[2], pin_out[2];
     DevBuf din[2], dout[2];
     size_t bytes() const {
-        return H.bytes + Hw.bytes + Hg.bytes + G.bytes + part.bytes + A.bytes + din[0].bytes + din[1].bytes +
-               dout[0].bytes + dout[1].bytes;
+        return H.bytes + Hw.bytes + Hg.bytes + G.bytes + part.bytes + A.bytes + bnd.bytes +
+               din[0].bytes + din[1].bytes + dout[0].bytes + dout[1].bytes;
     }
 };
 
@@ -388,6 +390,81 @@ struct DeviceGuard {
     }
 };
 
+// ------------------------------------------------------------------------------ exact range
+
+// The 128-bit walk computes the hafnian modulo 2^128 (it only adds and multiplies), so its
+// two's-complement result is the hafnian whenever |haf| < 2^127.  An instance is accepted only
+// when an upper bound proves that: log2 of min((K-1)!! * m^(K/2), sqrt(prod_i max(r_i, 1))),
+// m = max(1, max |p|), r_i = sum_{j != i} |p_ij| (for entrywise absolute values
+// haf^2 <= perm <= the product of the row sums).  The same bound covers every (K-2)-minor, so
+// it also covers the gradient.  The margin absorbs the rounding of the logarithms.
+constexpr double kExactLog2Limit = 127.0 - 1e-6;
+
+template <class PV>
+static double exact_log2_bound(int K, PV &&pv) {
+    double r[64] = {};
+    double m = 1.0;
+    int64_t q = 0;
+    for (int i = 0; i < K; i++)
+        for (int j = i + 1; j < K; j++, q++) {
+            const double a = std::fabs((double)pv(q));
+            r[i] += a;
+            r[j] += a;
+            m = std::max(m, a);
+        }
+    double lf = 0.5 * K * std::log2(m), lr = 0.0;
+    for (int k = K - 1; k > 1; k -= 2) lf += std::log2((double)k);
+    for (int i = 0; i < K; i++) lr += std::log2(std::max(r[i], 1.0));
+    return std::min(lf, 0.5 * lr);
+}
+
+static int exact_overflow(int64_t b, double lb) {
+    return fail(EGGMAN_EOVERFLOW,
+                "exact hafnian of instance %lld may need %.1f bits: the 128-bit walk is exact only "
+                "below 2^127 in magnitude",
+                (long long)(b + 1), lb + 1.0);
+}
+
+// Instances [0, B) of P ([pair][ld], int64 host memory); b_first numbers them in messages.
+static int check_exact_host(int K, const int64_t *P, int64_t ld, int64_t B, int64_t b_first = 0) {
+    for (int64_t b = 0; b < B; b++) {
+        const double lb = exact_log2_bound(K, [&](int64_t q) { return P[q * ld + b]; });
+        if (lb >= kExactLog2Limit) return exact_overflow(b_first + b, lb);
+    }
+    return EGGMAN_OK;
+}
+
+// The boson-sampling shape: instance b reads A[idx_b[i], idx_b[j]] (A column-major, N x N).
+static int check_exact_repeated(int K, const int64_t *A, int64_t N, const int32_t *idx, int64_t B) {
+    std::vector<int64_t> pv((size_t)K * (K - 1) / 2);
+    for (int64_t b = 0; b < B; b++) {
+        const int32_t *ix = idx + b * K;
+        int64_t q = 0;
+        for (int i = 0; i < K; i++)
+            for (int j = i + 1; j < K; j++) pv[q++] = A[(int64_t)(ix[j] - 1) * N + (ix[i] - 1)];
+        const double lb = exact_log2_bound(K, [&](int64_t q) { return pv[q]; });
+        if (lb >= kExactLog2Limit) return exact_overflow(b, lb);
+    }
+    return EGGMAN_OK;
+}
+
+// Device-resident P: the bounds are computed on the device (stream st), copied back, and the
+// stream is synchronised before the check.
+static int check_exact_device(DevState &ds, int K, const void *P, int64_t ldp, int64_t B,
+                              hipStream_t st, int64_t b_first = 0) {
+    if (B <= 0) return EGGMAN_OK;
+    EGG_TRY(ensure(ds.bnd, (size_t)B * sizeof(double)));
+    k_exact_log2_bound<<<dim3((unsigned)((B + 255) / 256)), dim3(256), 0, st>>>(
+        (const long long *)P, ldp, B, K, (double *)ds.bnd.p);
+    EGG_HIP(hipGetLastError());
+    std::vector<double> lb((size_t)B);
+    EGG_HIP(hipMemcpyAsync(lb.data(), ds.bnd.p, (size_t)B * sizeof(double), hipMemcpyDeviceToHost, st));
+    EGG_HIP(hipStreamSynchronize(st));
+    for (int64_t b = 0; b < B; b++)
+        if (lb[(size_t)b] >= kExactLog2Limit) return exact_overflow(b_first + b, lb[(size_t)b]);
+    return EGGMAN_OK;
+}
+
 // ------------------------------------------------------------------------------ tuning
 
 // Launch choices for one forward call.  Every choice gives bit-identical results.
@@ -484,10 +561,10 @@ static Tune pick_tune(const DevPlan &pl, int dtype, int64_t B) {
                    std::get<1>(i->first) == pl.lay.K && std::get<2>(i->first) == pl.lay.index_bytes &&
                    std::get<3>(i->first) == dtype;
         };
-        // a one-instance entry (one lane per state) says nothing about a batch: below 64
-        // instances without an entry of their own, the defaults (as many lanes per state as
-        // instances) serve it
-        if (B > 1 && B < 64) return default_tune(pl.lay, dtype, B);
+        // a one-instance entry (one lane per state) says nothing about a batch, nor a batch
+        // entry about one instance: below 64 instances without an entry of their own, the
+        // defaults (as many lanes per state as instances) serve it
+        if (B < 64) return default_tune(pl.lay, dtype, B);
         if (hi != g_tune.begin() && same(std::prev(hi))) it = std::prev(hi);
         else if (same(hi)) it = hi;
         else return default_tune(pl.lay, dtype, B);
@@ -1838,6 +1915,7 @@ int eggman_dp_forward(uint64_t plan, int dtype, const void *P_host, int64_t B, i
                     (long long)npairs, pl->lay.K, (long long)pl->lay.npairs);
     if (B == 0) return EGGMAN_OK;
     if (!P_host || !out_host) return fail(EGGMAN_EINVAL, "null P or out pointer");
+    if (dtype == EGGMAN_I128) EGG_TRY(check_exact_host(pl->lay.K, (const int64_t *)P_host, B, B));
     DeviceGuard g(pl->device);
     EGG_TRY(g.rc);
     DevState *ds;
@@ -1892,6 +1970,7 @@ int eggman_dp_forward_device(uint64_t plan, int dtype, const void *P_dev, int64_
     EGG_TRY(dev_state(pl->device, &ds));
     hipStream_t st = stream ? (hipStream_t)stream : ds->stream;
     EGG_TRY(chain_stream(*ds, st));
+    if (dtype == EGGMAN_I128) EGG_TRY(check_exact_device(*ds, pl->lay.K, P_dev, ldp, B, st));
     const Tune tune = pick_tune(*pl, dtype, B);
     return forward_any(dtype, *ds, *pl, P_dev, ldp, B, out_dev, st, tune);
 }
@@ -2084,6 +2163,7 @@ int eggman_dp_forward_repeated(uint64_t plan, int dtype, const void *A_host, int
         if (idx_host[t] < 1 || idx_host[t] > N)
             return fail(EGGMAN_EINVAL, "index list %lld holds %d, outside 1..%lld",
                         (long long)(t / K + 1), idx_host[t], (long long)N);
+    if (dtype == EGGMAN_I128) EGG_TRY(check_exact_repeated(K, (const int64_t *)A_host, N, idx_host, B));
     DeviceGuard g(pl->device);
     EGG_TRY(g.rc);
     DevState *ds;
@@ -2142,6 +2222,14 @@ int eggman_dp_forward_mixed(int ngroups, const uint64_t *plans, const int64_t *c
     if (!hb) return fail(EGGMAN_EINVAL, "unknown dtype %d", dtype);
     if (total == 0) return EGGMAN_OK;
     if (!P_host || !out_host) return fail(EGGMAN_EINVAL, "null P or out pointer");
+    if (dtype == EGGMAN_I128) {
+        int64_t off = 0, first = 0;
+        for (size_t gi = 0; gi < pls.size(); gi++) {
+            EGG_TRY(check_exact_host(pls[gi]->lay.K, (const int64_t *)P_host + off, cnt[gi], cnt[gi], first));
+            off += pls[gi]->lay.npairs * cnt[gi];
+            first += cnt[gi];
+        }
+    }
     DeviceGuard g(pls[0]->device);
     EGG_TRY(g.rc);
     DevState *ds;
@@ -2180,6 +2268,15 @@ int eggman_dp_forward_mixed_device(int ngroups, const uint64_t *plans, const int
     EGG_TRY(dev_state(pls[0]->device, &ds));
     hipStream_t st = stream ? (hipStream_t)stream : ds->stream;
     EGG_TRY(chain_stream(*ds, st));
+    if (dtype == EGGMAN_I128) {
+        int64_t off = 0, first = 0;
+        for (size_t gi = 0; gi < pls.size(); gi++) {
+            EGG_TRY(check_exact_device(*ds, pls[gi]->lay.K, (const int64_t *)P_dev + off, cnt[gi],
+                                       cnt[gi], st, first));
+            off += pls[gi]->lay.npairs * cnt[gi];
+            first += cnt[gi];
+        }
+    }
     return forward_mixed_any(dtype, *ds, pls, cnt, P_dev, out_dev, st);
 }
 
@@ -2196,6 +2293,7 @@ int eggman_dp_forward_grad(uint64_t plan, int dtype, const void *P_host, int64_t
                     (long long)npairs, pl->lay.K, (long long)pl->lay.npairs);
     if (B == 0) return EGGMAN_OK;
     if (!P_host || !out_host || !dP_host) return fail(EGGMAN_EINVAL, "null P, out or dP pointer");
+    if (dtype == EGGMAN_I128) EGG_TRY(check_exact_host(pl->lay.K, (const int64_t *)P_host, B, B));
     DeviceGuard g(pl->device);
     EGG_TRY(g.rc);
     DevState *ds;
@@ -2255,11 +2353,25 @@ int eggman_dp_forward_grad_device(uint64_t plan, int dtype, const void *P_dev, i
     EGG_TRY(dev_state(pl->device, &ds));
     hipStream_t st = stream ? (hipStream_t)stream : ds->stream;
     EGG_TRY(chain_stream(*ds, st));
+    if (dtype == EGGMAN_I128) EGG_TRY(check_exact_device(*ds, pl->lay.K, P_dev, ldp, B, st));
     EGG_TRY(ensure_rev(*pl));
     Tune tune;
     int64_t cap;
     EGG_TRY(grad_tune(*pl, dtype, B, &tune, &cap));
     return forward_grad_any(dtype, *ds, *pl, P_dev, ldp, B, out_dev, dP_dev, lddp, st, tune);
+}
+
+int eggman_exact_bound(int K, const int64_t *P, int64_t ldp, int64_t B, double *log2_bound) {
+    if (K < 2 || K > 64 || (K & 1) || B < 0 || ldp < B)
+        return fail(EGGMAN_EINVAL, "need an even degree 2..64 and 0 <= B <= ldp");
+    if (B > 0 && !P) return fail(EGGMAN_EINVAL, "null P");
+    int rc = EGGMAN_OK;
+    for (int64_t b = 0; b < B; b++) {
+        const double lb = exact_log2_bound(K, [&](int64_t q) { return P[q * ldp + b]; });
+        if (log2_bound) log2_bound[b] = lb;
+        if (lb >= kExactLog2Limit && rc == EGGMAN_OK) rc = exact_overflow(b, lb);
+    }
+    return rc;
 }
 
 int64_t eggman_cache_bytes(void) {
@@ -2280,8 +2392,8 @@ int eggman_cache_clear(void) {
         for (GraphEntry &ge : kv.second.graphs) (void)hipGraphExecDestroy(ge.exec);
         kv.second.graphs.clear();
         DevState &d = kv.second;
-        for (DevBuf *b : {&d.H, &d.Hw, &d.Hg, &d.G, &d.part, &d.A, &d.din[0], &d.din[1], &d.dout[0],
-                          &d.dout[1]}) {
+        for (DevBuf *b : {&d.H, &d.Hw, &d.Hg, &d.G, &d.part, &d.A, &d.bnd, &d.din[0], &d.din[1],
+                          &d.dout[0], &d.dout[1]}) {
             if (b->p) (void)hipFree(b->p);
             b->p = nullptr;
             b->bytes = 0;

@@ -706,6 +706,36 @@ __global__ __launch_bounds__(256) void k_gather_pairs(const PT *__restrict__ A, 
     }
 }
 
+// Range check of the exact (128-bit) walk, one thread per instance of P ([pair][ldp], int64):
+// out[b] = log2 of min((K-1)!! * m^(K/2), sqrt(prod_i max(r_i, 1))), m = max(1, max |p|) and
+// r_i = sum_{j != i} |p_ij|, an upper bound on |haf| and on every (K-2)-minor of instance b
+// (for entrywise absolute values haf^2 <= perm <= the product of the row sums).
+__global__ __launch_bounds__(256) void k_exact_log2_bound(const long long *__restrict__ P,
+                                                          int64_t ldp, int64_t B, int K,
+                                                          double *__restrict__ out) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    double m = 1.0, lr = 0.0;
+    for (int i = 0; i < K; i++) {
+        double ri = 0.0;
+        // pairs (j, i), j < i: q = j*K - j(j+1)/2 + (i - j - 1); pairs (i, j), j > i
+        for (int j = 0; j < i; j++) {
+            const int64_t q = (int64_t)j * K - (int64_t)j * (j + 1) / 2 + (i - j - 1);
+            ri += fabs((double)P[q * ldp + b]);
+        }
+        const int64_t q0 = (int64_t)i * K - (int64_t)i * (i + 1) / 2;
+        for (int j = i + 1; j < K; j++) {
+            const double a = fabs((double)P[(q0 + (j - i - 1)) * ldp + b]);
+            ri += a;
+            m = fmax(m, a);
+        }
+        lr += log2(fmax(ri, 1.0));
+    }
+    double lf = 0.5 * K * log2(m);
+    for (int k = K - 1; k > 1; k -= 2) lf += log2((double)k);
+    out[b] = fmin(lf, 0.5 * lr);
+}
+
 // Row of ones for the empty set (H[1] = one, src/dp.jl:421): a plain store kernel, so the
 // sequence stays valid inside a captured graph.
 template <int DT>

@@ -12,7 +12,7 @@ import threading
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libeggman_hip.so")
 
-OK, EINVAL, ENOMEM, EPLAN, EHIP, EUNSUPPORTED = 0, -1, -2, -3, -4, -5
+OK, EINVAL, ENOMEM, EPLAN, EHIP, EUNSUPPORTED, EOVERFLOW = 0, -1, -2, -3, -4, -5, -6
 F32, F64, C32, C64, I128 = 1, 2, 3, 4, 5
 
 
@@ -59,6 +59,7 @@ SIGNATURES = {
     "eggman_tune_set": (_int, [_int, _int, _int, _i64, _int, _i64, _int, _int]),
     "eggman_tune_get": (_int, [_int, _int, _int, _i64, ctypes.POINTER(_int), _i64p,
                                ctypes.POINTER(_int), ctypes.POINTER(_int)]),
+    "eggman_exact_bound": (_int, [_int, _vp, _i64, _i64, _vp]),
     "eggman_cache_bytes": (_i64, []),
     "eggman_cache_clear": (_int, []),
     "eggman_batch_bytes": (_i64, [_int, _int, _i64]),
@@ -103,4 +104,6 @@ def check(rc: int, what: str = "") -> None:
         raise MemoryError(msg)
     if rc == EUNSUPPORTED:
         raise NotImplementedError(msg)
+    if rc == EOVERFLOW:
+        raise OverflowError(msg)
     raise EggmanError(f"{what}: {msg}" if what else msg)

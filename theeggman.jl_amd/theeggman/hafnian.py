@@ -288,7 +288,9 @@ def hafnian_exact(A, *, check_symmetric: bool = True, backend=None,
     New entry point (the reference promotes integers to Float64, src/hafnian.jl:744).  The DP
     runs in 128-bit two's-complement integers, which is exact whenever the hafnian fits in 127
     bits -- for 0/1 adjacency matrices at every degree up to 64 with room to spare at N <= 40,
-    where (N-1)!! < 2^79.  Entries must be integers that fit in int64.
+    where (N-1)!! < 2^79.  Entries must be integers that fit in int64.  A matrix whose hafnian
+    is not provably below 2^127 in magnitude (bound: the smaller of (N-1)!! * max|a|^(N/2) and
+    sqrt(prod_i sum_{j != i} |a_ij|)) raises OverflowError before anything runs.
     """
     batch = _is_batch(A)
     As = [np.asarray(M) for M in A] if batch else [np.asarray(A)]
@@ -434,16 +436,26 @@ def hafnian_repeated_grad(A, rpt, *, check_symmetric: bool = True, backend=None)
 
 def rrule(f, A, *args, **kwargs):
     """ChainRulesCore-style reverse rule for ``hafnian``: returns ``(y, pullback)`` where
-    ``pullback(dy)`` gives ``(None, dy * G)`` with G from :func:`hafnian_grad`."""
+    ``pullback(dy)`` gives ``(None, dA)``, the cotangent of A for a function of a symmetric
+    matrix: ``dA = dy * conj(G) / 2`` with G from :func:`hafnian_grad` (G[i, j] = G[j, i] =
+    d haf / d a_ij for the pair variable a_ij; zero diagonal).
+
+    Under the Frobenius pairing a symmetric tangent dA then gives <dA_bar, dA> =
+    sum_{i<j} G_ij dA_ij, the directional derivative: the symmetric pair (i, j), (j, i) shares
+    one derivative, so each half carries G_ij / 2 (returning G itself would double every
+    symmetric chain such as X -> X + X^T).  For complex A the conjugate follows ChainRules'
+    convention for holomorphic functions (the cotangent is dy * conj(f')); for real A it is a
+    no-op."""
     if f is not hafnian:
         raise ArgumentError("rrule is defined for hafnian only")
     y, G = hafnian_grad(A, *args, **kwargs)
+    Gbar = np.conj(G) / 2
 
     def pullback(dy):
         dy = np.asarray(dy)
-        if G.ndim == 3:
-            return None, dy[:, None, None] * G
-        return None, dy * G
+        if Gbar.ndim == 3:
+            return None, dy[:, None, None] * Gbar
+        return None, dy * Gbar
 
     return y, pullback
 
