@@ -137,6 +137,23 @@ int eggman_dp_forward_device(uint64_t plan, int dtype, const void *P_dev, int64_
                              int64_t B, void *out_dev, void *stream);
 
 /*
+ * eggman_dp_forward_device without any allocation.  No counterpart in the reference: for callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  The caller passes P and out as for
+ * eggman_dp_forward_device; the state levels live in one static device array of 128 MiB per GPU
+ * (held from the library's first use of that GPU until the process ends), and the call makes no
+ * hipMalloc/hipFree, captures no graph and, after its first call on a device (which looks up the
+ * array's address), creates nothing.  Pass sizes are fitted to the array (the values are the same
+ * bits as eggman_dp_forward_device); a degree whose two largest levels of one instance do not fit
+ * is refused with EGGMAN_ENOMEM (e.g. complex N >= 36), and EGGMAN_I128 with EGGMAN_EUNSUPPORTED.
+ * Every call on one GPU shares the array and leaves its contents behind: the library orders its
+ * own calls, but work that overlaps outside that order on one GPU (replays of graphs captured from
+ * these calls on different streams) races.  Asynchronous on `stream` (NULL = the library's).
+ */
+int eggman_dp_forward_device_noalloc(uint64_t plan, int dtype, const void *P_dev, int64_t ldp,
+                                     int64_t B, void *out_dev, void *stream);
+
+/*
  * The boson-sampling batch shape, hafnian_repeated(A, rpts; backend) (src/hafnian.jl:1317-1359):
  * one matrix A (N x N, column-major as Julia stores it; int64 entries for EGGMAN_I128) and B index
  * lists idx_host[b][0..K-1] (int32, 1-based rows of A, repeats allowed, K = the plan's degree).

@@ -115,6 +115,7 @@ struct DevState {
     DevBuf part;                     // per-segment partial gradients
     DevBuf A;                        // staged matrix (repeated-pattern batches)
     DevBuf bnd;                      // per-instance range bounds of exact device-pointer calls
+    void *scratch_dev = nullptr;     // this device's address of the no-allocation scratch array
     // host entry points: a two-slot pipeline of pinned host buffers and device buffers, with an
     // upload and a download stream beside the compute stream (run_pipe)
     hipStream_t up = nullptr, down = nullptr;
@@ -481,6 +482,7 @@ struct Tune {
     bool vec = false;   // LDS kernel with 16-byte loads: 8 lanes per state, 8 states per wave
     bool deep = false;  // with vec: child loads in batches of 6 transitions instead of 4
     int64_t sub = 0;    // gradient: instances per forward sub-pass inside a pass (0 = the pass)
+    bool whole_small = true; // walk the small end levels once for the whole batch (extra buffers)
 };
 
 // Tuning flags as they cross the C ABI (eggman_tune_set / eggman_tune_get).
@@ -854,25 +856,110 @@ static void launch_fused_range(const DevCaps &caps, const DevPlan &pl, int La, i
 constexpr int64_t kSmallLevel = 4096;
 constexpr size_t kSmallLevelBytes = size_t(1) << 30;
 
+// How a forward call of B instances walks its levels, and the state memory it needs.
+struct WalkPlan {
+    bool fused = false;        // one fused launch (states in LDS), no state memory
+    int64_t chunk = 0;         // instances per pass
+    bool pad = false;          // level rows padded to whole 128-byte tile rows (16-byte loads)
+    int64_t rows = 0, ldmax = 0, ldw = 0;
+    int lb = 1, lt = 0;        // levels 2..lb and lt..top walk once for the whole batch
+    int fb = 1, ft = 0;        // one-pass walks: levels 2..fb and ft..top in one fused launch each
+    int64_t wrows_b = 0, wrows_t = 0;
+    size_t h_bytes = 0;        // two levels of `ldmax`-wide rows (ping-pong)
+    size_t w_bytes = 0;        // whole-batch rows of the small bottom and top levels
+};
+
+static bool fused_fits(const DevCaps &caps, const PlanLayout &lay, int dtype) {
+    return fused_lds_bytes(lay, dtype) <= caps.max_lds_block && lay.nlevels - 1 <= kFusedMaxLevels &&
+           lay.nstates <= INT32_MAX;
+}
+
+static void walk_plan(const DevCaps &caps, const PlanLayout &lay, const Tune &tune, int64_t B, int dtype,
+                      WalkPlan *w) {
+    *w = WalkPlan();
+    const int top = lay.K / 2 + 1;
+    w->lt = w->ft = top + 1;
+    if (tune.fused && fused_fits(caps, lay, dtype)) {
+        w->fused = true;
+        return;
+    }
+    const size_t hb = (size_t)elt_hbytes(dtype);
+    w->chunk = tune.chunk > 0 ? std::min(tune.chunk, B) : B;
+    w->rows = std::max<int64_t>(lay.max_level, 1);
+    // level rows: nb apart, or for the 16-byte-load kernel padded to whole 128-byte tile rows
+    w->pad = tune.bt == 64 && tune.lds && tune.vec;
+    w->ldmax = w->pad ? pad_rows(w->chunk) : w->chunk;
+    w->ldw = w->pad ? pad_rows(B) : B;
+    // Small levels at the bottom (2 .. lb) and the top (lt .. top) are walked once for the whole
+    // batch; the passes walk lb+1 .. lt-1, reading level lb's whole-batch rows at their columns
+    // and writing level lt-1's.  Small levels are bound by latency (few states, long sums), so
+    // one launch over every instance costs about what one pass's launch did.
+    if (w->chunk < B && tune.bt == 64 && tune.whole_small) {
+        int lb = 1, lt = top + 1;
+        int64_t wb = 0, wt = 0;
+        while (lb + 1 < top && lay.nst[lb] <= kSmallLevel) lb++;
+        while (lt - 1 > lb + 1 && lay.nst[lt - 2] <= kSmallLevel) lt--;
+        for (int L = 1; L <= lb && lb > 1; L++) wb = std::max(wb, lay.nst[L - 1]);
+        for (int L = lt - 1; L < top && lt <= top; L++) wt = std::max(wt, lay.nst[L - 1]);
+        if ((size_t)2 * (wb + wt) * w->ldw * hb <= kSmallLevelBytes) {
+            w->lb = lb;
+            w->lt = lt;
+            w->wrows_b = wb;
+            w->wrows_t = wt;
+        }
+    }
+    // one pass of few instances: the run of small levels 2 .. fb and the run ft .. top each walk
+    // in one fused launch (runs of one level gain nothing and are not fused)
+    if (w->chunk == B && tune.bt < 64 && lay.nstates < INT32_MAX && top <= kFusedMaxLevels) {
+        int fb = 1, ft = top + 1;
+        while (fb + 1 <= top && lay.nst[fb] <= kFuseLevel && range_lds_bytes(lay, 2, fb + 1, dtype) <= kFuseLds)
+            fb++;
+        if (fb == 2) fb = 1;
+        if (fb < top) {
+            while (ft - 1 > fb + 1 && lay.nst[ft - 2] <= kFuseLevel &&
+                   range_lds_bytes(lay, ft - 1, top, dtype) <= kFuseLds)
+                ft--;
+            if (ft == top) ft = top + 1;
+        }
+        w->fb = fb;
+        w->ft = ft;
+    }
+    w->h_bytes = ((size_t)2 * w->rows * w->ldmax * hb + 255) & ~(size_t)255;
+    w->w_bytes = (size_t)2 * (w->wrows_b + w->wrows_t) * w->ldw * hb;
+}
+
+// Caller-provided state memory (the no-allocation entry point): forward_levels then carves its
+// level buffers from it instead of the library's grow-only buffers, and allocates nothing.
+struct WorkSpan {
+    char *p = nullptr;
+    size_t bytes = 0;
+};
+
 // Walk all levels for instances [0, B) of P (rows ldp apart) into out, `chunk` instances per
 // pass, keeping only two levels of state values (the level being written and the level below,
 // which holds every child it reads).
 template <int DT, typename W>
 static int forward_levels(DevState &ds, const DevPlan &pl, const void *P, int64_t ldp, int64_t B,
-                          void *out, hipStream_t st, const Tune &tune) {
+                          void *out, hipStream_t st, const Tune &tune, const WorkSpan *ws = nullptr) {
     using E = Elt<DT>;
     const PlanLayout &lay = pl.lay;
-    if (tune.fused && launch_fused<DT, W>(ds.caps, pl, P, ldp, B, out, st)) {
+    WalkPlan w;
+    walk_plan(ds.caps, lay, tune, B, DT, &w);
+    if (w.fused && launch_fused<DT, W>(ds.caps, pl, P, ldp, B, out, st)) {
         EGG_HIP(hipGetLastError());
         return EGGMAN_OK;
     }
-    const int64_t chunk = tune.chunk > 0 ? std::min(tune.chunk, B) : B;
+    if (w.fused) {   // the kernel declined after all: walk level by level as the tuning says
+        Tune t = tune;
+        t.fused = false;
+        walk_plan(ds.caps, lay, t, B, DT, &w);
+    }
+    const int64_t chunk = w.chunk;
     const size_t hb = sizeof(typename E::H), pb = sizeof(typename E::P);
-    const int64_t rows = std::max<int64_t>(lay.max_level, 1);
-    // level rows: nb apart, or for the 16-byte-load kernel padded to whole 128-byte tile rows
-    const bool pad = tune.bt == 64 && tune.lds && tune.vec;
-    const int64_t ldmax = pad ? pad_rows(chunk) : chunk;
+    const bool pad = w.pad;
+    const int64_t ldw = w.ldw;
     const int top = lay.K / 2 + 1;
+    const int lb = w.lb, lt = w.lt, fb = w.fb, ft = w.ft;
     auto launch = [&](int L, const char *Pc, const void *Hc, int64_t ldh, void *Ho, int64_t ldo,
                       int64_t nb) {
         if (tune.bt == 64 && tune.lds &&
@@ -880,48 +967,28 @@ static int forward_levels(DevState &ds, const DevPlan &pl, const void *P, int64_
             return;
         launch_level_bt<DT, W>(tune.bt, pl, L, Pc, ldp, Hc, ldh, Ho, ldo, nb, st);
     };
-    // Small levels at the bottom (2 .. lb) and the top (lt .. top) are walked once for the whole
-    // batch; the passes walk lb+1 .. lt-1, reading level lb's whole-batch rows at their columns
-    // and writing level lt-1's.  Small levels are bound by latency (few states, long sums), so
-    // one launch over every instance costs about what one pass's launch did.
-    int lb = 1, lt = top + 1;
-    const int64_t ldw = pad ? pad_rows(B) : B;
-    int64_t wrows_b = 0, wrows_t = 0;
-    if (chunk < B && tune.bt == 64) {
-        while (lb + 1 < top && lay.nst[lb] <= kSmallLevel) lb++;
-        while (lt - 1 > lb + 1 && lay.nst[lt - 2] <= kSmallLevel) lt--;
-        for (int L = 1; L <= lb && lb > 1; L++) wrows_b = std::max(wrows_b, lay.nst[L - 1]);
-        for (int L = lt - 1; L < top && lt <= top; L++) wrows_t = std::max(wrows_t, lay.nst[L - 1]);
-        if ((size_t)2 * (wrows_b + wrows_t) * ldw * hb > kSmallLevelBytes) {
-            lb = 1;
-            lt = top + 1;
-            wrows_b = wrows_t = 0;
+    char *hmem, *wmem = nullptr;
+    if (ws) {
+        if (w.h_bytes + w.w_bytes > ws->bytes)
+            return fail(EGGMAN_ENOMEM, "the walk needs %.1f MiB of state memory, %.1f MiB are provided",
+                        (w.h_bytes + w.w_bytes) / 1048576.0, ws->bytes / 1048576.0);
+        hmem = ws->p;
+        wmem = ws->p + w.h_bytes;
+    } else {
+        EGG_TRY(ensure(ds.H, w.h_bytes));
+        hmem = (char *)ds.H.p;
+        if (w.w_bytes > 0) {
+            EGG_TRY(ensure(ds.Hw, w.w_bytes));
+            wmem = (char *)ds.Hw.p;
         }
     }
-    // one pass of few instances: the run of small levels 2 .. fb and the run ft .. top each walk
-    // in one fused launch (runs of one level gain nothing and are not fused)
-    int fb = 1, ft = top + 1;
-    if (chunk == B && tune.bt < 64 && lay.nstates < INT32_MAX && top <= kFusedMaxLevels) {
-        while (fb + 1 <= top && lay.nst[fb] <= kFuseLevel && range_lds_bytes(lay, 2, fb + 1, DT) <= kFuseLds)
-            fb++;
-        if (fb == 2) fb = 1;
-        if (fb < top) {
-            while (ft - 1 > fb + 1 && lay.nst[ft - 2] <= kFuseLevel &&
-                   range_lds_bytes(lay, ft - 1, top, DT) <= kFuseLds)
-                ft--;
-            if (ft == top) ft = top + 1;
-        }
-    }
-    EGG_TRY(ensure(ds.H, (size_t)2 * rows * ldmax * hb));
-    char *buf[2] = {(char *)ds.H.p, (char *)ds.H.p + (size_t)rows * ldmax * hb};
+    char *buf[2] = {hmem, hmem + (size_t)w.rows * w.ldmax * hb};
     char *wb[2] = {nullptr, nullptr}, *wt[2] = {nullptr, nullptr};
-    if (wrows_b + wrows_t > 0) {
-        EGG_TRY(ensure(ds.Hw, (size_t)2 * (wrows_b + wrows_t) * ldw * hb));
-        char *w = (char *)ds.Hw.p;
-        wb[0] = w;
-        wb[1] = w + (size_t)wrows_b * ldw * hb;
-        wt[0] = w + (size_t)2 * wrows_b * ldw * hb;
-        wt[1] = wt[0] + (size_t)wrows_t * ldw * hb;
+    if (w.w_bytes > 0) {
+        wb[0] = wmem;
+        wb[1] = wmem + (size_t)w.wrows_b * ldw * hb;
+        wt[0] = wmem + (size_t)2 * w.wrows_b * ldw * hb;
+        wt[1] = wt[0] + (size_t)w.wrows_t * ldw * hb;
     }
     if (lb > 1) {
         k_fill_one<DT><<<dim3((unsigned)((B + 255) / 256)), dim3(256), 0, st>>>(
@@ -965,19 +1032,20 @@ static int forward_levels(DevState &ds, const DevPlan &pl, const void *P, int64_
 
 template <int DT>
 static int forward_dt(DevState &ds, const DevPlan &pl, const void *P, int64_t ldp, int64_t B,
-                      void *out, hipStream_t st, const Tune &tune) {
-    if (pl.lay.index_bytes == 4) return forward_levels<DT, int32_t>(ds, pl, P, ldp, B, out, st, tune);
-    return forward_levels<DT, int64_t>(ds, pl, P, ldp, B, out, st, tune);
+                      void *out, hipStream_t st, const Tune &tune, const WorkSpan *ws) {
+    if (pl.lay.index_bytes == 4) return forward_levels<DT, int32_t>(ds, pl, P, ldp, B, out, st, tune, ws);
+    return forward_levels<DT, int64_t>(ds, pl, P, ldp, B, out, st, tune, ws);
 }
 
 static int forward_eager(int dtype, DevState &ds, const DevPlan &pl, const void *P, int64_t ldp,
-                         int64_t B, void *out, hipStream_t st, const Tune &tune) {
+                         int64_t B, void *out, hipStream_t st, const Tune &tune,
+                         const WorkSpan *ws = nullptr) {
     switch (dtype) {
-    case EGGMAN_F64: return forward_dt<EGGMAN_F64>(ds, pl, P, ldp, B, out, st, tune);
-    case EGGMAN_F32: return forward_dt<EGGMAN_F32>(ds, pl, P, ldp, B, out, st, tune);
-    case EGGMAN_C64: return forward_dt<EGGMAN_C64>(ds, pl, P, ldp, B, out, st, tune);
-    case EGGMAN_C32: return forward_dt<EGGMAN_C32>(ds, pl, P, ldp, B, out, st, tune);
-    case EGGMAN_I128: return forward_dt<EGGMAN_I128>(ds, pl, P, ldp, B, out, st, tune);
+    case EGGMAN_F64: return forward_dt<EGGMAN_F64>(ds, pl, P, ldp, B, out, st, tune, ws);
+    case EGGMAN_F32: return forward_dt<EGGMAN_F32>(ds, pl, P, ldp, B, out, st, tune, ws);
+    case EGGMAN_C64: return forward_dt<EGGMAN_C64>(ds, pl, P, ldp, B, out, st, tune, ws);
+    case EGGMAN_C32: return forward_dt<EGGMAN_C32>(ds, pl, P, ldp, B, out, st, tune, ws);
+    case EGGMAN_I128: return forward_dt<EGGMAN_I128>(ds, pl, P, ldp, B, out, st, tune, ws);
     }
     return fail(EGGMAN_EINVAL, "unknown dtype %d", dtype);
 }
@@ -1030,6 +1098,53 @@ static int forward_graph(int dtype, DevState &ds, const DevPlan &pl, const void 
     }
     ds.graphs.push_back(ge);
     return EGGMAN_OK;   // this call's results came from the eager pass
+}
+
+// ------------------------------------------------------------------------------ no-allocation walk
+
+// State memory of eggman_dp_forward_device_noalloc: one array per device, held from the library's
+// first use of that device until the process ends.  On the host the array's name is not a device
+// address: host code reaches the memory only through the address hipGetSymbolAddress returns for
+// the current device (DevState::scratch_dev), looked up once per device.
+constexpr size_t kScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static int scratch_address(DevState &ds, void **p) {
+    if (!ds.scratch_dev) {
+        void *a = nullptr;
+        EGG_HIP(hipGetSymbolAddress(&a, HIP_SYMBOL(scratch)));
+        ds.scratch_dev = a;
+    }
+    *p = ds.scratch_dev;
+    return EGGMAN_OK;
+}
+
+// Launch choices of the no-allocation walk: the tuned choices without a captured graph, fitted to
+// the scratch array -- first without the whole-batch rows of the small end levels, then with the
+// pass halved until its two levels of rows fit (below 64 instances per pass, lanes per state
+// follow the pass).  Every choice gives the same bits.  Fails if one instance per pass does not
+// fit.
+static int noalloc_tune(const DevState &ds, const DevPlan &pl, int dtype, int64_t B, Tune *out) {
+    Tune t = pick_tune(pl, dtype, B);
+    t.graph = false;
+    WalkPlan w;
+    for (;;) {
+        walk_plan(ds.caps, pl.lay, t, B, dtype, &w);
+        if (w.fused || w.h_bytes + w.w_bytes <= kScratchBytes) break;
+        if (w.w_bytes > 0 && t.whole_small) {
+            t.whole_small = false;
+            continue;
+        }
+        if (w.chunk <= 1)
+            return fail(EGGMAN_ENOMEM,
+                        "degree %d needs %.1f MiB of state memory for one instance; the "
+                        "no-allocation entry point holds %zu MiB (use eggman_dp_forward_device)",
+                        pl.lay.K, w.h_bytes / 1048576.0, kScratchBytes >> 20);
+        t.chunk = (w.chunk + 1) / 2;
+        if (t.chunk < 64) t.bt = pow2_at_least(t.chunk);
+    }
+    *out = t;
+    return EGGMAN_OK;
 }
 
 static int forward_any(int dtype, DevState &ds, const DevPlan &pl, const void *P, int64_t ldp,
@@ -1973,6 +2088,34 @@ int eggman_dp_forward_device(uint64_t plan, int dtype, const void *P_dev, int64_
     if (dtype == EGGMAN_I128) EGG_TRY(check_exact_device(*ds, pl->lay.K, P_dev, ldp, B, st));
     const Tune tune = pick_tune(*pl, dtype, B);
     return forward_any(dtype, *ds, *pl, P_dev, ldp, B, out_dev, st, tune);
+}
+
+int eggman_dp_forward_device_noalloc(uint64_t plan, int dtype, const void *P_dev, int64_t ldp,
+                                     int64_t B, void *out_dev, void *stream) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    DevPlan *pl;
+    EGG_TRY(find_plan(plan, &pl));
+    if (!elt_hbytes(dtype)) return fail(EGGMAN_EINVAL, "unknown dtype %d", dtype);
+    if (dtype == EGGMAN_I128)
+        return fail(EGGMAN_EUNSUPPORTED, "the exact walk checks its range through host memory and "
+                                         "a synchronisation; the no-allocation entry point serves "
+                                         "the floating-point types (use eggman_dp_forward_device)");
+    if (B < 0 || ldp < B) return fail(EGGMAN_EINVAL, "need 0 <= B <= ldp (B=%lld, ldp=%lld)",
+                                      (long long)B, (long long)ldp);
+    if (B == 0) return EGGMAN_OK;
+    if (!P_dev || !out_dev) return fail(EGGMAN_EINVAL, "null P or out pointer");
+    DeviceGuard g(pl->device);
+    EGG_TRY(g.rc);
+    DevState *ds;
+    EGG_TRY(dev_state(pl->device, &ds));
+    void *sp;
+    EGG_TRY(scratch_address(*ds, &sp));
+    Tune tune;
+    EGG_TRY(noalloc_tune(*ds, *pl, dtype, B, &tune));
+    hipStream_t st = stream ? (hipStream_t)stream : ds->stream;
+    EGG_TRY(chain_stream(*ds, st));
+    const WorkSpan ws{(char *)sp, kScratchBytes};
+    return forward_eager(dtype, *ds, *pl, P_dev, ldp, B, out_dev, st, tune, &ws);
 }
 
 // Time every launch choice for (plan, dtype, B) on synthetic pair values and keep the fastest.
