@@ -169,15 +169,17 @@ int eggman_dp_forward_repeated(uint64_t plan, int dtype, const void *A_host, int
  * Hafnians of instances of different degrees in one device pass.  Group g has counts[g]
  * instances sharing plan handle plans[g]; P_host is the concatenation of the groups'
  * [pair][counts[g]] blocks (group 0 first) and out_host receives the groups' results in the same
- * order.  Groups of 256 or more instances, which fill the GPU by themselves, walk one after
- * another with their own tuned launch; the smaller groups share one walk in which every launch
- * advances each of them by one level.  All plans must live on one device and use one index
- * width.  Results are bit-identical to
- * eggman_dp_forward on each group alone.  New: the reference requires one size per batch
- * (src/hafnian.jl:1277-1280).
+ * order.  Groups of 256 or more instances, which fill the GPU by themselves, each walk with their
+ * own tuned launch on a stream of their own, concurrently with one another and with the shared
+ * walk of the smaller groups, in which every launch advances each of them by one level.  All
+ * plans must live on one device and use one index width.  The call is processed in chunks that
+ * take the same share of every group: enough to keep the staging within the free device memory,
+ * with at most max_batch instances of any one group in a chunk (<= 0: no limit).  Results are bit-identical to
+ * eggman_dp_forward on each group alone, for every chunking.  New: the reference requires one
+ * size per batch (src/hafnian.jl:1277-1280).
  */
 int eggman_dp_forward_mixed(int ngroups, const uint64_t *plans, const int64_t *counts, int dtype,
-                            const void *P_host, void *out_host);
+                            const void *P_host, int64_t max_batch, void *out_host);
 
 /* Device-resident form of eggman_dp_forward_mixed, asynchronous on `stream`. */
 int eggman_dp_forward_mixed_device(int ngroups, const uint64_t *plans, const int64_t *counts,
@@ -240,6 +242,10 @@ int eggman_autotune(uint64_t plan, int dtype, int64_t B, int reps, char *report,
                                  to whole 128-byte tile rows) */
 #define EGGMAN_TUNE_DEEP 128  /* with EGGMAN_TUNE_VEC: child loads in batches of 6 transitions
                                  instead of 4 */
+#define EGGMAN_TUNE_PAIRS 256 /* with EGGMAN_TUNE_VEC: an 8-lane slot takes two parents {p} u T,
+                                 {p+1} u T that share their children, loading them once (a
+                                 quarter fewer child and plan-word loads); the per-level item
+                                 list is built on first use */
 
 /*
  * Set / read a tuning entry directly (e.g. to persist tuned choices across processes).  bt:

@@ -72,3 +72,25 @@ def test_mixed_large_and_small_groups():
         assert got[idx].tobytes() == same.tobytes(), n
     P = synth.pair_batch(np.stack([As[i] for i in np.flatnonzero(sizes == 20)]))
     assert got[sizes == 20].tobytes() == o.haf_dp(P, 20, o.F64, 8).tobytes()
+
+
+def test_mixed_chunking_is_bit_identical():
+    # the host entry point cuts a call into chunks that take the same share of every group
+    # (max_batch instances of a group per chunk, ragged shares included); concurrent large groups
+    # and the shared walk of the small ones give the same bits for every chunking, and the exact
+    # type is chunked the same way
+    note("mixed chunking")
+    sizes = np.array([16] * 270 + [20] * 300 + [12] * 7 + [24] * 33)
+    As = [synth.sym_uniform(int(n), 45, i) for i, n in enumerate(sizes)]
+    whole = te.hafnian_mixed(As, check_symmetric=False)
+    for mb in (1, 7, 64, 299):
+        got = te.hafnian_mixed(As, check_symmetric=False, max_batch=mb)
+        assert got.tobytes() == whole.tobytes(), mb
+    Gs = [synth.gnp_adjacency(n, 46, i) for i, n in enumerate([12] * 5 + [16] * 3)]
+    Ps = [te._gather_pairs_batch(np.int64, np.stack(Gs[:5]), None, 12),
+          te._gather_pairs_batch(np.int64, np.stack(Gs[5:]), None, 16)]
+    plans = [te._dp_plan(12), te._dp_plan(16)]
+    a = te._haf_dp_backend_mixed(te.HIPBackend(), Ps, plans)
+    b = te._haf_dp_backend_mixed(te.HIPBackend(), Ps, plans, max_batch=2)
+    assert a == b
+    assert a[1] == o.haf_dp(synth.pair_batch(np.stack(Gs[5:])).astype(np.int64), 16, o.I128)

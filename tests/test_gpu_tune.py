@@ -42,7 +42,10 @@ def test_autotune_choice_is_bit_exact(K, B):
                                      dict(bt=64, chunk=0, lds=1, occ=4, wide=1, vec=1),
                                      dict(bt=64, chunk=0, lds=1, occ=2, graph=1, vec=1),
                                      dict(bt=64, chunk=64, lds=1, occ=2, vec=1, deep=1),
-                                     dict(bt=64, chunk=0, lds=1, occ=4, wide=1, vec=1, deep=1)])
+                                     dict(bt=64, chunk=0, lds=1, occ=4, wide=1, vec=1, deep=1),
+                                     dict(bt=64, chunk=64, lds=1, occ=2, vec=1, pairs=1),
+                                     dict(bt=64, chunk=0, lds=1, occ=2, graph=1, vec=1, pairs=1),
+                                     dict(bt=64, chunk=64, lds=1, occ=4, wide=1, vec=1, deep=1, pairs=1)])
 def test_every_launch_choice_gives_the_same_bits(setting):
     K, B = 18, 150
     te.tune_set(K, np.float64, B, **setting)
@@ -82,7 +85,8 @@ def test_tuning_table_round_trip(tmp_path):
     assert te.load_tuning(str(path)) >= 1
     assert te.tune_get(22, np.float64, 512) == {"bt": 64, "chunk": 256, "flags": 1 | 8, "occ": 2,
                                                 "lds": 1, "graph": 0, "fused": 0, "wide": 1,
-                                                "ckpt": 0, "narrow": 0, "vec": 0, "deep": 0}
+                                                "ckpt": 0, "narrow": 0, "vec": 0, "deep": 0,
+                                                "pairs": 0}
 
 
 @pytest.mark.parametrize("K,B,T", [(28, 1024, np.float64), (24, 37, np.float64), (22, 130, np.complex128),
@@ -146,18 +150,26 @@ def test_vector_loads_give_the_same_bits(K, B, T, chunk):
     # batches of 6 transitions instead of 4 (every remainder of k mod 6 occurs over the levels)
     te.tune_set(K, T, B, bt=64, chunk=chunk, occ=2, lds=True, vec=True, deep=True, index_bytes=ib)
     deep = te._haf_dp_backend(BE, P, plan)
+    # parent pairs sharing their children, in batches of 4 and of 6
+    te.tune_set(K, T, B, bt=64, chunk=chunk, occ=2, lds=True, vec=True, pairs=True, index_bytes=ib)
+    pairs = te._haf_dp_backend(BE, P, plan)
+    te.tune_set(K, T, B, bt=64, chunk=chunk, occ=2, lds=True, vec=True, deep=True, pairs=True,
+                index_bytes=ib)
+    pairs6 = te._haf_dp_backend(BE, P, plan)
     if T == np.int64:
-        assert list(got) == list(wide) == list(deep)
+        assert list(got) == list(wide) == list(deep) == list(pairs) == list(pairs6)
     else:
         assert np.asarray(got).tobytes() == np.asarray(wide).tobytes()
         assert np.asarray(deep).tobytes() == np.asarray(wide).tobytes()
+        assert np.asarray(pairs).tobytes() == np.asarray(wide).tobytes()
+        assert np.asarray(pairs6).tobytes() == np.asarray(wide).tobytes()
         if K <= 24:
             code = {np.float64: o.F64, np.float32: o.F32, np.complex128: o.C64, np.complex64: o.C32}[T]
             want = o.haf_dp(synth.pair_batch(As), K, code, 8)
             assert np.asarray(got).tobytes() == want.tobytes()
     if T != np.int64 and B <= 300:
         # the gradient's forward pass on padded level rows: the same bits as without
-        _, g_vec = te._haf_dp_backend_grad(BE, P, plan)     # (the deep entry is still set)
+        _, g_vec = te._haf_dp_backend_grad(BE, P, plan)     # (the deep pairs entry is still set)
         te.tune_set(K, T, B, bt=64, chunk=chunk, occ=2, lds=True, index_bytes=ib)
         _, g_wide = te._haf_dp_backend_grad(BE, P, plan)
         assert np.asarray(g_vec).tobytes() == np.asarray(g_wide).tobytes()

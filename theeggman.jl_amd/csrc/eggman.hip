@@ -1,4 +1,11 @@
 // eggman HIP library: device caches, level drivers and the C ABI of include/eggman_hip.h.
+//
+// The file is compiled once per part, in parallel: EGG_PART = 0 holds the shared state and the
+// C ABI; EGG_PART = EGGMAN_F32 .. EGGMAN_I128 each instantiate the level drivers (and so the
+// kernels) of one element type.  Without EGG_PART the whole library is one translation unit.
+#ifndef EGG_PART
+#define EGG_PART (-1)
+#endif
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,6 +32,7 @@ namespace eggman {
 
 // ------------------------------------------------------------------------------ errors
 
+#if EGG_PART <= 0
 static thread_local std::string t_err;
 
 int fail(int code, const char *fmt, ...) {
@@ -36,6 +44,7 @@ int fail(int code, const char *fmt, ...) {
     t_err = buf;
     return code;
 }
+#endif
 
 // ------------------------------------------------------------------------------ state
 
@@ -61,6 +70,12 @@ struct DevPlan {
     int64_t part_rows = 0, nred = 0;
     size_t rev_bytes = 0;
     GradLevels gl_full, gl_ckpt;   // where the gradient keeps forward levels (plan_host.hpp)
+    // parent-pair items of each level (k_level_fwd_lds_vec2), built on first use: level L's
+    // items at d_items + item_off[L-1], nitems[L-1] of them
+    mutable bool items_ready = false;
+    mutable uint32_t *d_items = nullptr;
+    mutable std::vector<int64_t> item_off, nitems;
+    mutable size_t items_bytes = 0;
 };
 
 // Grow-only device buffer (ext/TheEggmanKernelAbstractionsExt.jl:68-88).
@@ -93,7 +108,11 @@ struct GraphEntry {
 };
 
 // Bumped whenever a work buffer is reallocated, which invalidates captured graphs.
-static uint64_t g_buf_gen = 1;
+#if EGG_PART <= 0
+uint64_t g_buf_gen = 1;
+#else
+extern uint64_t g_buf_gen;
+#endif
 
 // Grow-only pinned host buffer (hipHostMalloc), the staging memory of the host entry points.
 struct PinBuf {
@@ -115,6 +134,11 @@ struct DevState {
     DevBuf part;                     // per-segment partial gradients
     DevBuf A;                        // staged matrix (repeated-pattern batches)
     DevBuf bnd;                      // per-instance range bounds of exact device-pointer calls
+    DevBuf Hm;                       // the walks of a mixed call's large groups, one slice each
+    // side streams of mixed calls (default priority), forked from and joined to the call's stream
+    static constexpr int kSide = 4;
+    hipStream_t side[kSide] = {};
+    hipEvent_t ev_fork = nullptr, ev_join[kSide] = {};
     void *scratch_dev = nullptr;     // this device's address of the no-allocation scratch array
     // host entry points: a two-slot pipeline of pinned host buffers and device buffers, with an
     // upload and a download stream beside the compute stream (run_pipe)
@@ -123,16 +147,24 @@ struct DevState {
     PinBuf pin_in[2], pin_out[2];
     DevBuf din[2], dout[2];
     size_t bytes() const {
-        return H.bytes + Hw.bytes + Hg.bytes + G.bytes + part.bytes + A.bytes + bnd.bytes +
+        return H.bytes + Hw.bytes + Hg.bytes + G.bytes + part.bytes + A.bytes + bnd.bytes + Hm.bytes +
                din[0].bytes + din[1].bytes + dout[0].bytes + dout[1].bytes;
     }
 };
 
-static std::recursive_mutex g_mu;
-static std::map<uint64_t, std::unique_ptr<DevPlan>> g_plans;
-static std::map<std::tuple<int, int, int>, uint64_t> g_plan_key;   // (device, K, index_bytes)
-static uint64_t g_next_id = 1;
-static std::map<int, DevState> g_dev;
+#if EGG_PART <= 0
+std::recursive_mutex g_mu;
+std::map<uint64_t, std::unique_ptr<DevPlan>> g_plans;
+std::map<std::tuple<int, int, int>, uint64_t> g_plan_key;   // (device, K, index_bytes)
+uint64_t g_next_id = 1;
+std::map<int, DevState> g_dev;
+#else
+extern std::recursive_mutex g_mu;
+extern std::map<uint64_t, std::unique_ptr<DevPlan>> g_plans;
+extern std::map<std::tuple<int, int, int>, uint64_t> g_plan_key;
+extern uint64_t g_next_id;
+extern std::map<int, DevState> g_dev;
+#endif
 
 static int current_device(int *dev) {
     EGG_HIP(hipGetDevice(dev));
@@ -483,18 +515,19 @@ struct Tune {
     bool deep = false;  // with vec: child loads in batches of 6 transitions instead of 4
     int64_t sub = 0;    // gradient: instances per forward sub-pass inside a pass (0 = the pass)
     bool whole_small = true; // walk the small end levels once for the whole batch (extra buffers)
+    bool pairs = false; // with vec: slots take parent pairs sharing their children (k_level_fwd_lds_vec2)
 };
 
 // Tuning flags as they cross the C ABI (eggman_tune_set / eggman_tune_get).
 enum {
     TUNE_LDS = 1, TUNE_GRAPH = 2, TUNE_FUSED = 4, TUNE_WIDE = 8, TUNE_CKPT = 16, TUNE_NARROW = 32,
-    TUNE_VEC = 64, TUNE_DEEP = 128,
-    TUNE_ALL = 255
+    TUNE_VEC = 64, TUNE_DEEP = 128, TUNE_PAIRS = 256,
+    TUNE_ALL = 511
 };
 static int tune_flags(const Tune &t) {
     return (t.lds ? TUNE_LDS : 0) | (t.graph ? TUNE_GRAPH : 0) | (t.fused ? TUNE_FUSED : 0) |
            (t.wide ? TUNE_WIDE : 0) | (t.ckpt ? TUNE_CKPT : 0) | (t.narrow ? TUNE_NARROW : 0) |
-           (t.vec ? TUNE_VEC : 0) | (t.deep ? TUNE_DEEP : 0);
+           (t.vec ? TUNE_VEC : 0) | (t.deep ? TUNE_DEEP : 0) | (t.pairs ? TUNE_PAIRS : 0);
 }
 static Tune tune_from_flags(int bt, int64_t chunk, int flags, int occ) {
     Tune t;
@@ -509,6 +542,7 @@ static Tune tune_from_flags(int bt, int64_t chunk, int flags, int occ) {
     t.narrow = (flags & TUNE_NARROW) != 0;
     t.vec = (flags & TUNE_VEC) != 0;
     t.deep = (flags & TUNE_DEEP) != 0;
+    t.pairs = (flags & TUNE_PAIRS) != 0;
     return t;
 }
 
@@ -543,7 +577,11 @@ static Tune default_tune(const PlanLayout &lay, int, int64_t B) {
 // Tuned choices per (device, degree, index width, element type, batch rounded up to a power of
 // two), written by eggman_autotune or eggman_tune_set.
 using TuneKey = std::tuple<int, int, int, int, int64_t>;
-static std::map<TuneKey, Tune> g_tune;
+#if EGG_PART <= 0
+std::map<TuneKey, Tune> g_tune;
+#else
+extern std::map<TuneKey, Tune> g_tune;
+#endif
 
 static int64_t batch_bucket(int64_t B) {
     int64_t b = 1;
@@ -578,6 +616,64 @@ static Tune pick_tune(const DevPlan &pl, int dtype, int64_t B) {
 }
 
 // ------------------------------------------------------------------------------ forward
+
+// The parent-pair items of every level (k_level_fwd_lds_vec2), built once per plan from its own
+// device transitions: per level the flags of k_item_flags, compacted on the host into item words
+// (state number in the level | second parent follows << 31).
+static int ensure_items(const DevPlan &pl) {
+    if (pl.items_ready) return EGGMAN_OK;
+    const PlanLayout &lay = pl.lay;
+    if (lay.nstates >= (int64_t(1) << 31))
+        return fail(EGGMAN_EUNSUPPORTED, "parent-pair items need fewer than 2^31 states");
+    const int top = lay.K / 2 + 1;
+    std::vector<int64_t> off(top + 1, 0), cnt(top + 1, 0);
+    std::vector<uint32_t> items;
+    std::vector<uint8_t> flags((size_t)std::max<int64_t>(lay.max_level, 1));
+    uint8_t *d_flags = nullptr;
+    EGG_HIP(hipMalloc(&d_flags, flags.size()));
+    int rc = EGGMAN_OK;
+    for (int L = 2; L <= top && rc == EGGMAN_OK; L++) {
+        const int64_t nst = lay.nst[L - 1];
+        const int k = lay.size[L - 1] - 1;
+        const unsigned blocks = (unsigned)((nst + 255) / 256);
+        if (lay.index_bytes == 4)
+            k_item_flags<int32_t><<<dim3(blocks), dim3(256)>>>((const int32_t *)pl.d_trans + lay.tstart[L - 1],
+                                                              nst, k, lay.K, lay.lo[L - 1], d_flags);
+        else
+            k_item_flags<int64_t><<<dim3(blocks), dim3(256)>>>((const int64_t *)pl.d_trans + lay.tstart[L - 1],
+                                                              nst, k, lay.K, lay.lo[L - 1], d_flags);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpy(flags.data(), d_flags, (size_t)nst, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) {
+            rc = fail(EGGMAN_EHIP, "item flags: %s", hipGetErrorString(e));
+            break;
+        }
+        off[L - 1] = (int64_t)items.size();
+        for (int64_t r = 0; r < nst; r++)
+            if (flags[(size_t)r] & 1) items.push_back((uint32_t)r | ((flags[(size_t)r] & 2) ? 0x80000000u : 0u));
+        cnt[L - 1] = (int64_t)items.size() - off[L - 1];
+    }
+    (void)hipFree(d_flags);
+    if (rc != EGGMAN_OK) return rc;
+    uint32_t *d = nullptr;
+    const size_t bytes = std::max<size_t>(items.size(), 1) * sizeof(uint32_t);
+    hipError_t e = hipMalloc(&d, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(EGGMAN_ENOMEM, "parent-pair items: %s", hipGetErrorString(e));
+    }
+    e = hipMemcpy(d, items.data(), items.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return fail(EGGMAN_EHIP, "parent-pair items: %s", hipGetErrorString(e));
+    }
+    pl.d_items = d;
+    pl.items_bytes = bytes;
+    pl.item_off = off;
+    pl.nitems = cnt;
+    pl.items_ready = true;
+    return EGGMAN_OK;
+}
 
 template <int DT, typename W, int BT>
 static void launch_level(const DevPlan &pl, int L, const void *P, int64_t ldp, const void *Hc,
@@ -637,10 +733,10 @@ static void level_pairs(const PlanLayout &lay, int L, int *qlo, int *nq) {
 
 // The LDS level kernels: 64-instance rows, one state per wave (KIND_WIDE); 16-instance tiles,
 // 4 states per wave (KIND_NARROW); 16-byte loads, 8 states per wave (KIND_VEC).
-enum { KIND_WIDE = 0, KIND_NARROW = 1, KIND_VEC = 2 };
+enum { KIND_WIDE = 0, KIND_NARROW = 1, KIND_VEC = 2, KIND_VEC2 = 3 };
 
 template <int DT> constexpr int kind_tile(int kind) {
-    return kind == KIND_VEC ? 128 / (int)sizeof(typename Elt<DT>::H) : kind == KIND_NARROW ? 16 : 64;
+    return kind >= KIND_VEC ? 128 / (int)sizeof(typename Elt<DT>::H) : kind == KIND_NARROW ? 16 : 64;
 }
 
 // The 16-byte-load kernel reads whole 128-byte tile rows: it needs rows a multiple of its tile
@@ -668,7 +764,8 @@ static bool launch_level_lds_w(const DevCaps &caps, const Tune &tune, const DevP
     level_pairs(lay, L, &qlo, &nq);
     const size_t lds = (size_t)nq * TW * sizeof(typename E::P);
     if (lds > caps.max_lds_block) return false;
-    auto kern = KIND == KIND_VEC      ? k_level_fwd_lds_vec<DT, W, WPB, TAG, UB>
+    auto kern = KIND == KIND_VEC2     ? k_level_fwd_lds_vec2<DT, W, WPB, TAG, UB>
+                : KIND == KIND_VEC    ? k_level_fwd_lds_vec<DT, W, WPB, TAG, UB>
                 : KIND == KIND_NARROW ? k_level_fwd_lds_narrow<DT, W, WPB, TAG>
                                       : k_level_fwd_lds<DT, W, WPB, TAG>;
     static bool attr_set = false;
@@ -681,8 +778,9 @@ static bool launch_level_lds_w(const DevCaps &caps, const Tune &tune, const DevP
         attr_set = true;
     }
     const int64_t nst = lay.nst[L - 1];
-    const LdsMap map = make_lds_map(caps, tune.occ, nst, nb, lds, WPB, TW,
-                                    KIND == KIND_VEC ? 8 : KIND == KIND_NARROW ? 4 : 1);
+    const int64_t nwork = KIND == KIND_VEC2 ? pl.nitems[L - 1] : nst;   // items or states
+    const LdsMap map = make_lds_map(caps, tune.occ, nwork, nb, lds, WPB, TW,
+                                    KIND >= KIND_VEC ? 8 : KIND == KIND_NARROW ? 4 : 1);
     LevelArgs<DT, W> a;
     a.trans_lvl = (const W *)pl.d_trans + lay.tstart[L - 1];
     a.nst = nst;
@@ -699,6 +797,10 @@ static bool launch_level_lds_w(const DevCaps &caps, const Tune &tune, const DevP
     a.nb = nb;
     a.map = map;
     a.addr32 = child_addr32(lay.nstates, ldh * (int64_t)sizeof(typename E::H));
+    a.items = KIND == KIND_VEC2 ? pl.d_items + pl.item_off[L - 1] : nullptr;
+    a.nitems = KIND == KIND_VEC2 ? pl.nitems[L - 1] : 0;
+    a.K = lay.K;
+    a.plo = lay.lo[L - 1];
     kern<<<dim3((unsigned)(8 * map.per_xcd)), dim3(WPB * 64), lds, st>>>(a);
     return true;
 }
@@ -707,6 +809,16 @@ template <int DT, typename W, int TAG = 0>
 static bool launch_level_lds(const DevCaps &caps, const Tune &tune, const DevPlan &pl, int L,
                              const void *P, int64_t ldp, const void *Hc, int64_t ldh, void *Ho,
                              int64_t ldo, int64_t nb, hipStream_t st) {
+    if (tune.vec && tune.pairs && pl.items_ready && vec_rows_ok<DT>(Hc, ldh)) {
+        if (tune.deep) {
+            if (tune.wide)
+                return launch_level_lds_w<DT, W, 16, TAG, KIND_VEC2, 6>(caps, tune, pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st);
+            return launch_level_lds_w<DT, W, kLdsWaves, TAG, KIND_VEC2, 6>(caps, tune, pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st);
+        }
+        if (tune.wide)
+            return launch_level_lds_w<DT, W, 16, TAG, KIND_VEC2>(caps, tune, pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st);
+        return launch_level_lds_w<DT, W, kLdsWaves, TAG, KIND_VEC2>(caps, tune, pl, L, P, ldp, Hc, ldh, Ho, ldo, nb, st);
+    }
     if (tune.vec && vec_rows_ok<DT>(Hc, ldh)) {
         if (tune.deep) {
             if (tune.wide)
@@ -967,6 +1079,7 @@ static int forward_levels(DevState &ds, const DevPlan &pl, const void *P, int64_
             return;
         launch_level_bt<DT, W>(tune.bt, pl, L, Pc, ldp, Hc, ldh, Ho, ldo, nb, st);
     };
+    if (tune.pairs) EGG_TRY(ensure_items(pl));
     char *hmem, *wmem = nullptr;
     if (ws) {
         if (w.h_bytes + w.w_bytes > ws->bytes)
@@ -1031,11 +1144,23 @@ static int forward_levels(DevState &ds, const DevPlan &pl, const void *P, int64_
 }
 
 template <int DT>
-static int forward_dt(DevState &ds, const DevPlan &pl, const void *P, int64_t ldp, int64_t B,
+int forward_dt(DevState &ds, const DevPlan &pl, const void *P, int64_t ldp, int64_t B,
                       void *out, hipStream_t st, const Tune &tune, const WorkSpan *ws) {
     if (pl.lay.index_bytes == 4) return forward_levels<DT, int32_t>(ds, pl, P, ldp, B, out, st, tune, ws);
     return forward_levels<DT, int64_t>(ds, pl, P, ldp, B, out, st, tune, ws);
 }
+
+// One part of the library instantiates this for its element type; the others use that.
+#if EGG_PART >= 0
+extern template int forward_dt<EGGMAN_F32>(DevState &, const DevPlan &, const void *, int64_t, int64_t, void *, hipStream_t, const Tune &, const WorkSpan *);
+extern template int forward_dt<EGGMAN_F64>(DevState &, const DevPlan &, const void *, int64_t, int64_t, void *, hipStream_t, const Tune &, const WorkSpan *);
+extern template int forward_dt<EGGMAN_C32>(DevState &, const DevPlan &, const void *, int64_t, int64_t, void *, hipStream_t, const Tune &, const WorkSpan *);
+extern template int forward_dt<EGGMAN_C64>(DevState &, const DevPlan &, const void *, int64_t, int64_t, void *, hipStream_t, const Tune &, const WorkSpan *);
+extern template int forward_dt<EGGMAN_I128>(DevState &, const DevPlan &, const void *, int64_t, int64_t, void *, hipStream_t, const Tune &, const WorkSpan *);
+#endif
+#if EGG_PART > 0
+template int forward_dt<EGG_PART>(DevState &, const DevPlan &, const void *, int64_t, int64_t, void *, hipStream_t, const Tune &, const WorkSpan *);
+#endif
 
 static int forward_eager(int dtype, DevState &ds, const DevPlan &pl, const void *P, int64_t ldp,
                          int64_t B, void *out, hipStream_t st, const Tune &tune,
@@ -1100,6 +1225,7 @@ static int forward_graph(int dtype, DevState &ds, const DevPlan &pl, const void 
     return EGGMAN_OK;   // this call's results came from the eager pass
 }
 
+#if EGG_PART <= 0
 // ------------------------------------------------------------------------------ no-allocation walk
 
 // State memory of eggman_dp_forward_device_noalloc: one array per device, held from the library's
@@ -1127,6 +1253,7 @@ static int scratch_address(DevState &ds, void **p) {
 static int noalloc_tune(const DevState &ds, const DevPlan &pl, int dtype, int64_t B, Tune *out) {
     Tune t = pick_tune(pl, dtype, B);
     t.graph = false;
+    if (!pl.items_ready) t.pairs = false;   // building the items would allocate (same bits)
     WalkPlan w;
     for (;;) {
         walk_plan(ds.caps, pl.lay, t, B, dtype, &w);
@@ -1146,6 +1273,8 @@ static int noalloc_tune(const DevState &ds, const DevPlan &pl, int dtype, int64_
     *out = t;
     return EGGMAN_OK;
 }
+
+#endif
 
 static int forward_any(int dtype, DevState &ds, const DevPlan &pl, const void *P, int64_t ldp,
                        int64_t B, void *out, hipStream_t st, const Tune &tune) {
@@ -1295,7 +1424,7 @@ static int forward_mixed_w(DevState &ds, const std::vector<DevPlan *> &pls,
 constexpr int64_t kMixedSolo = 256;
 
 template <int DT>
-static int forward_mixed_dt(DevState &ds, const std::vector<DevPlan *> &pls,
+int forward_mixed_dt(DevState &ds, const std::vector<DevPlan *> &pls,
                             const std::vector<int64_t> &counts, const void *P, void *out,
                             hipStream_t st) {
     using HT = typename Elt<DT>::H;
@@ -1325,11 +1454,51 @@ static int forward_mixed_dt(DevState &ds, const std::vector<DevPlan *> &pls,
             sout.push_back(outv[g]);
         }
     }
-    std::stable_sort(solo.begin(), solo.end(),
-                     [&](size_t a, size_t b) { return pls[a]->lay.nstates > pls[b]->lay.nstates; });
+    // the large groups walk concurrently, each on a side stream with its own slice of ds.Hm (the
+    // largest plans first, on separate streams), beside the shared walk of the small groups on st
+    std::stable_sort(solo.begin(), solo.end(), [&](size_t a, size_t b) {
+        return (double)pls[a]->lay.ntrans * counts[a] > (double)pls[b]->lay.ntrans * counts[b];
+    });
+    std::vector<Tune> tunes;
+    std::vector<size_t> span;
+    size_t total = 0;
     for (size_t g : solo) {
-        const Tune t = pick_tune(*pls[g], DT, counts[g]);
-        EGG_TRY(forward_any(DT, ds, *pls[g], Pv[g], counts[g], counts[g], outv[g], st, t));
+        Tune t = pick_tune(*pls[g], DT, counts[g]);
+        t.graph = false;
+        if (t.pairs) EGG_TRY(ensure_items(*pls[g]));
+        WalkPlan w;
+        walk_plan(ds.caps, pls[g]->lay, t, counts[g], DT, &w);
+        const size_t bytes = (w.h_bytes + w.w_bytes + 255) & ~(size_t)255;
+        tunes.push_back(t);
+        span.push_back(bytes);
+        total += bytes;
+    }
+    if (!solo.empty()) {
+        EGG_TRY(ensure(ds.Hm, std::max<size_t>(total, 256)));
+        if (!ds.ev_fork) {
+            EGG_HIP(hipEventCreateWithFlags(&ds.ev_fork, hipEventDisableTiming));
+            for (int i = 0; i < DevState::kSide; i++) {
+                EGG_HIP(hipStreamCreateWithFlags(&ds.side[i], hipStreamNonBlocking));
+                EGG_HIP(hipEventCreateWithFlags(&ds.ev_join[i], hipEventDisableTiming));
+            }
+        }
+        EGG_HIP(hipEventRecord(ds.ev_fork, st));
+        const int nside = (int)std::min<size_t>(solo.size(), DevState::kSide);
+        for (int i = 0; i < nside; i++) EGG_HIP(hipStreamWaitEvent(ds.side[i], ds.ev_fork, 0));
+        size_t off = 0;
+        int rc = EGGMAN_OK;
+        for (size_t i = 0; i < solo.size() && rc == EGGMAN_OK; i++) {
+            const size_t g = solo[i];
+            const WorkSpan ws{(char *)ds.Hm.p + off, span[i]};
+            rc = forward_eager(DT, ds, *pls[g], Pv[g], counts[g], counts[g], outv[g],
+                               ds.side[i % DevState::kSide], tunes[i], &ws);
+            off += span[i];
+        }
+        for (int i = 0; i < nside; i++) {
+            EGG_HIP(hipEventRecord(ds.ev_join[i], ds.side[i]));
+            EGG_HIP(hipStreamWaitEvent(st, ds.ev_join[i], 0));
+        }
+        EGG_TRY(rc);
     }
     if (spl.empty()) return EGGMAN_OK;
     size_t big = 0;
@@ -1341,6 +1510,18 @@ static int forward_mixed_dt(DevState &ds, const std::vector<DevPlan *> &pls,
         return forward_mixed_w<DT, int32_t>(ds, spl, scnt, sP, sout, st, t.wide, kind);
     return forward_mixed_w<DT, int64_t>(ds, spl, scnt, sP, sout, st, t.wide, kind);
 }
+
+// One part of the library instantiates this for its element type; the others use that.
+#if EGG_PART >= 0
+extern template int forward_mixed_dt<EGGMAN_F32>(DevState &, const std::vector<DevPlan *> &, const std::vector<int64_t> &, const void *, void *, hipStream_t);
+extern template int forward_mixed_dt<EGGMAN_F64>(DevState &, const std::vector<DevPlan *> &, const std::vector<int64_t> &, const void *, void *, hipStream_t);
+extern template int forward_mixed_dt<EGGMAN_C32>(DevState &, const std::vector<DevPlan *> &, const std::vector<int64_t> &, const void *, void *, hipStream_t);
+extern template int forward_mixed_dt<EGGMAN_C64>(DevState &, const std::vector<DevPlan *> &, const std::vector<int64_t> &, const void *, void *, hipStream_t);
+extern template int forward_mixed_dt<EGGMAN_I128>(DevState &, const std::vector<DevPlan *> &, const std::vector<int64_t> &, const void *, void *, hipStream_t);
+#endif
+#if EGG_PART > 0
+template int forward_mixed_dt<EGG_PART>(DevState &, const std::vector<DevPlan *> &, const std::vector<int64_t> &, const void *, void *, hipStream_t);
+#endif
 
 static int forward_mixed_any(int dtype, DevState &ds, const std::vector<DevPlan *> &pls,
                              const std::vector<int64_t> &counts, const void *P, void *out,
@@ -1635,6 +1816,7 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
     using HT = typename E::H;
     const PlanLayout &lay = pl.lay;
     EGG_TRY(ensure_rev(pl));
+    if (tune.pairs) EGG_TRY(ensure_items(pl));
     const size_t hb = sizeof(HT), pb = sizeof(typename E::P);
     const int64_t chunk = tune.chunk > 0 ? std::min(tune.chunk, B) : B;
     const bool par = sizeof(W) == 4 && use_par_dp(ds.caps, pl, DT);
@@ -1737,12 +1919,24 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
 }
 
 template <int DT>
-static int forward_grad_dt(DevState &ds, DevPlan &pl, const void *P, int64_t ldp, int64_t B,
+int forward_grad_dt(DevState &ds, DevPlan &pl, const void *P, int64_t ldp, int64_t B,
                            void *out, void *dP, int64_t lddp, hipStream_t st, const Tune &tune) {
     if (pl.lay.index_bytes == 4)
         return forward_grad_levels<DT, int32_t>(ds, pl, P, ldp, B, out, dP, lddp, st, tune);
     return forward_grad_levels<DT, int64_t>(ds, pl, P, ldp, B, out, dP, lddp, st, tune);
 }
+
+// One part of the library instantiates this for its element type; the others use that.
+#if EGG_PART >= 0
+extern template int forward_grad_dt<EGGMAN_F32>(DevState &, DevPlan &, const void *, int64_t, int64_t, void *, void *, int64_t, hipStream_t, const Tune &);
+extern template int forward_grad_dt<EGGMAN_F64>(DevState &, DevPlan &, const void *, int64_t, int64_t, void *, void *, int64_t, hipStream_t, const Tune &);
+extern template int forward_grad_dt<EGGMAN_C32>(DevState &, DevPlan &, const void *, int64_t, int64_t, void *, void *, int64_t, hipStream_t, const Tune &);
+extern template int forward_grad_dt<EGGMAN_C64>(DevState &, DevPlan &, const void *, int64_t, int64_t, void *, void *, int64_t, hipStream_t, const Tune &);
+extern template int forward_grad_dt<EGGMAN_I128>(DevState &, DevPlan &, const void *, int64_t, int64_t, void *, void *, int64_t, hipStream_t, const Tune &);
+#endif
+#if EGG_PART > 0
+template int forward_grad_dt<EGG_PART>(DevState &, DevPlan &, const void *, int64_t, int64_t, void *, void *, int64_t, hipStream_t, const Tune &);
+#endif
 
 static int forward_grad_any(int dtype, DevState &ds, DevPlan &pl, const void *P, int64_t ldp,
                             int64_t B, void *out, void *dP, int64_t lddp, hipStream_t st,
@@ -1857,13 +2051,25 @@ static int alloc_plan(DevPlan &p) {
 // ------------------------------------------------------------------------------ device gather
 
 template <typename PT>
-static void launch_gather(const void *A, int64_t N, const int32_t *idx, int K, int64_t B, void *P,
+void launch_gather(const void *A, int64_t N, const int32_t *idx, int K, int64_t B, void *P,
                           int64_t ldp, hipStream_t st) {
     const int64_t total = (int64_t)K * (K - 1) / 2 * B;
     const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (total + 255) / 256));
     k_gather_pairs<PT><<<dim3((unsigned)blocks), dim3(256), 0, st>>>((const PT *)A, N, idx, K, B,
                                                                       (PT *)P, ldp);
 }
+
+// One part of the library instantiates this for its element type; the others use that.
+#if EGG_PART >= 0
+extern template void launch_gather<float>(const void *, int64_t, const int32_t *, int, int64_t, void *, int64_t, hipStream_t);
+extern template void launch_gather<double>(const void *, int64_t, const int32_t *, int, int64_t, void *, int64_t, hipStream_t);
+extern template void launch_gather<float2>(const void *, int64_t, const int32_t *, int, int64_t, void *, int64_t, hipStream_t);
+extern template void launch_gather<double2>(const void *, int64_t, const int32_t *, int, int64_t, void *, int64_t, hipStream_t);
+extern template void launch_gather<long long>(const void *, int64_t, const int32_t *, int, int64_t, void *, int64_t, hipStream_t);
+#endif
+#if EGG_PART > 0
+template void launch_gather<typename Elt<EGG_PART>::P>(const void *, int64_t, const int32_t *, int, int64_t, void *, int64_t, hipStream_t);
+#endif
 
 static void gather_any(int dtype, const void *A, int64_t N, const int32_t *idx, int K, int64_t B,
                        void *P, int64_t ldp, hipStream_t st) {
@@ -1882,6 +2088,7 @@ using namespace eggman;
 
 // ================================================================================ C ABI
 
+#if EGG_PART <= 0
 extern "C" {
 
 const char *eggman_version(void) { return EGGMAN_VERSION_STRING; }
@@ -2163,20 +2370,22 @@ int eggman_autotune(uint64_t plan, int dtype, int64_t B, int reps, char *report,
                 t.wide = true;
                 cands.push_back(t);
             }
-            for (int occ : {1, 2, 4}) {
-                Tune t;
-                t.chunk = c;
-                t.occ = occ;
-                t.vec = true;
-                cands.push_back(t);
-                t.wide = true;
-                cands.push_back(t);
-                t.wide = false;
-                t.deep = true;
-                cands.push_back(t);
-                t.wide = true;
-                cands.push_back(t);
-            }
+            for (int occ : {1, 2, 4})
+                for (bool pairs : {false, true}) {
+                    Tune t;
+                    t.chunk = c;
+                    t.occ = occ;
+                    t.vec = true;
+                    t.pairs = pairs;
+                    cands.push_back(t);
+                    t.wide = true;
+                    cands.push_back(t);
+                    t.wide = false;
+                    t.deep = true;
+                    cands.push_back(t);
+                    t.wide = true;
+                    cands.push_back(t);
+                }
             Tune g;
             g.chunk = c;
             g.lds = false;
@@ -2355,7 +2564,7 @@ int eggman_dp_forward_repeated(uint64_t plan, int dtype, const void *A_host, int
 }
 
 int eggman_dp_forward_mixed(int ngroups, const uint64_t *plans, const int64_t *counts, int dtype,
-                            const void *P_host, void *out_host) {
+                            const void *P_host, int64_t max_batch, void *out_host) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     std::vector<DevPlan *> pls;
     std::vector<int64_t> cnt;
@@ -2365,32 +2574,86 @@ int eggman_dp_forward_mixed(int ngroups, const uint64_t *plans, const int64_t *c
     if (!hb) return fail(EGGMAN_EINVAL, "unknown dtype %d", dtype);
     if (total == 0) return EGGMAN_OK;
     if (!P_host || !out_host) return fail(EGGMAN_EINVAL, "null P or out pointer");
-    if (dtype == EGGMAN_I128) {
-        int64_t off = 0, first = 0;
-        for (size_t gi = 0; gi < pls.size(); gi++) {
-            EGG_TRY(check_exact_host(pls[gi]->lay.K, (const int64_t *)P_host + off, cnt[gi], cnt[gi], first));
-            off += pls[gi]->lay.npairs * cnt[gi];
-            first += cnt[gi];
-        }
+    const size_t ng = pls.size();
+    std::vector<int64_t> poff(ng), ooff(ng);   // each group's block in P_host and out_host
+    for (size_t g = 0, po = 0, oo = 0; g < ng; g++) {
+        poff[g] = (int64_t)po;
+        ooff[g] = (int64_t)oo;
+        po += (size_t)pls[g]->lay.npairs * cnt[g];
+        oo += (size_t)cnt[g];
     }
+    if (dtype == EGGMAN_I128)
+        for (size_t g = 0; g < ng; g++)
+            EGG_TRY(check_exact_host(pls[g]->lay.K, (const int64_t *)P_host + poff[g], cnt[g], cnt[g],
+                                     ooff[g]));
     DeviceGuard g(pls[0]->device);
     EGG_TRY(g.rc);
     DevState *ds;
     EGG_TRY(dev_state(pls[0]->device, &ds));
-    // one chunk: every group advances together in one device pass
+    // Chunks take the same share of every group (chunk c: instances [c*n/nc, (c+1)*n/nc) of a
+    // group of n), so each keeps all degrees together: enough chunks to keep two slots of staging
+    // within half the free device memory and one slot's pinned staging within kPipeSlotBytes, and
+    // at most max_batch instances of any group in a chunk.  Every instance's value is the same
+    // bits however it is chunked.
+    const size_t staged = (size_t)np_total * pb + (size_t)total * hb;
+    size_t free_b = 0, total_b = 0;
+    EGG_HIP(hipMemGetInfo(&free_b, &total_b));
+    int64_t maxcnt = 0;
+    for (int64_t c : cnt) maxcnt = std::max(maxcnt, c);
+    int64_t nchunks = 1;
+    nchunks = std::max<int64_t>(nchunks, (int64_t)((4 * staged + free_b - 1) / std::max<size_t>(free_b, 1)));
+    nchunks = std::max<int64_t>(nchunks, (int64_t)((staged + kPipeSlotBytes - 1) / kPipeSlotBytes));
+    nchunks = std::min(nchunks, maxcnt);
+    if (max_batch > 0) nchunks = std::max<int64_t>(nchunks, (maxcnt + max_batch - 1) / max_batch);
+    auto share = [&](int64_t c, size_t gi, int64_t *lo, int64_t *n) {
+        *lo = c * cnt[gi] / nchunks;
+        *n = (c + 1) * cnt[gi] / nchunks - *lo;
+    };
+    size_t in_max = 0, out_max = 0;
+    for (int64_t c = 0; c < nchunks; c++) {
+        size_t ib = 0, ob = 0;
+        for (size_t gi = 0; gi < ng; gi++) {
+            int64_t lo, n;
+            share(c, gi, &lo, &n);
+            ib += (size_t)pls[gi]->lay.npairs * n * pb;
+            ob += (size_t)n * hb;
+        }
+        in_max = std::max(in_max, ib);
+        out_max = std::max(out_max, ob);
+    }
     return run_pipe(
-        *ds, 1, (size_t)np_total * pb, (size_t)total * hb,
-        [&](int64_t, void *pin, size_t *ib, size_t *ob) {
-            copy_bytes(pin, P_host, (size_t)np_total * pb);
-            *ib = (size_t)np_total * pb;
-            *ob = (size_t)total * hb;
+        *ds, nchunks, in_max, out_max,
+        [&](int64_t c, void *pin, size_t *ib, size_t *ob) {
+            size_t o = 0, r = 0;
+            for (size_t gi = 0; gi < ng; gi++) {
+                int64_t lo, n;
+                share(c, gi, &lo, &n);
+                copy_columns((char *)pin + o, (const char *)P_host + (size_t)poff[gi] * pb,
+                             pls[gi]->lay.npairs, cnt[gi], lo, n, pb);
+                o += (size_t)pls[gi]->lay.npairs * n * pb;
+                r += (size_t)n * hb;
+            }
+            *ib = o;
+            *ob = r;
             return EGGMAN_OK;
         },
-        [&](int64_t, void *din, void *dout, hipStream_t st) {
-            return forward_mixed_any(dtype, *ds, pls, cnt, din, dout, st);
+        [&](int64_t c, void *din, void *dout, hipStream_t st) {
+            std::vector<int64_t> nc(ng);
+            for (size_t gi = 0; gi < ng; gi++) {
+                int64_t lo;
+                share(c, gi, &lo, &nc[gi]);
+            }
+            return forward_mixed_any(dtype, *ds, pls, nc, din, dout, st);
         },
-        [&](int64_t, const void *pin) {
-            std::memcpy(out_host, pin, (size_t)total * hb);
+        [&](int64_t c, const void *pin) {
+            size_t r = 0;
+            for (size_t gi = 0; gi < ng; gi++) {
+                int64_t lo, n;
+                share(c, gi, &lo, &n);
+                std::memcpy((char *)out_host + (size_t)(ooff[gi] + lo) * hb, (const char *)pin + r,
+                            (size_t)n * hb);
+                r += (size_t)n * hb;
+            }
             return EGGMAN_OK;
         });
 }
@@ -2522,6 +2785,7 @@ int64_t eggman_cache_bytes(void) {
     int64_t total = 0;
     for (auto &kv : g_plans) total += (int64_t)kv.second->bytes;
     for (auto &kv : g_plans) total += (int64_t)kv.second->rev_bytes;
+    for (auto &kv : g_plans) total += (int64_t)kv.second->items_bytes;
     for (auto &kv : g_dev) total += (int64_t)kv.second.bytes();
     return total;
 }
@@ -2535,7 +2799,7 @@ int eggman_cache_clear(void) {
         for (GraphEntry &ge : kv.second.graphs) (void)hipGraphExecDestroy(ge.exec);
         kv.second.graphs.clear();
         DevState &d = kv.second;
-        for (DevBuf *b : {&d.H, &d.Hw, &d.Hg, &d.G, &d.part, &d.A, &d.bnd, &d.din[0], &d.din[1],
+        for (DevBuf *b : {&d.H, &d.Hw, &d.Hg, &d.G, &d.part, &d.A, &d.bnd, &d.Hm, &d.din[0], &d.din[1],
                           &d.dout[0], &d.dout[1]}) {
             if (b->p) (void)hipFree(b->p);
             b->p = nullptr;
@@ -2552,7 +2816,7 @@ int eggman_cache_clear(void) {
         DevPlan &p = *kv.second;
         for (void *ptr : {p.d_trans, p.d_cent, p.d_coff, p.d_pent, p.d_segs, p.d_qoff, p.d_qseg,
                           p.d_prank, p.d_tasks, p.d_roff, p.d_rtask, p.d_qrow, p.d_qcol,
-                          p.d_red, p.d_srow0, p.d_sruns})
+                          p.d_red, p.d_srow0, p.d_sruns, (void *)p.d_items})
             if (ptr) (void)hipFree(ptr);
     }
     g_plans.clear();
@@ -2588,3 +2852,5 @@ int eggman_grad_levels(int K, int checkpointed, int64_t *kept_rows, int64_t *seg
 }
 
 }  // extern "C"
+#endif
+

@@ -176,6 +176,11 @@ struct LevelArgs {
     // child addresses fit 32-bit arithmetic: every packed child number is below 2^24 and
     // (child number + 1) * ldh * sizeof(H) below 2^32 (checked by the launcher)
     bool addr32;
+    // parent pairs (k_level_fwd_lds_vec2 only): the level's items, item i = state number within
+    // the level | (second parent follows) << 31; the degree and the level's smallest min(S)
+    const uint32_t *items;
+    int64_t nitems;
+    int K, plo;
 };
 
 // Launch-time check behind LevelArgs::addr32 and the other kernels' 32-bit address paths: a
@@ -511,6 +516,181 @@ __device__ inline void level_lds_vec_body(const LevelArgs<DT, W> &a, int64_t j,
     else level_lds_vec_body_a<DT, W, WPB, false, UB>(a, j, lds);
 }
 
+// ---------------------------------------------------------------------------------------
+// Parents that share their children.  A state S = {p} u T (p = min S) has the children
+// T \ {j}, j in T ascending, which depend on T only; the states {p} u T of one T are consecutive
+// in the plan (colex order), with p = p_lo, p_lo + 1, ... from the level's smallest p_lo.  Items
+// pair them up -- (p_lo, p_lo + 1), (p_lo + 2, p_lo + 3), ... -- and the 16-byte-load kernel below
+// walks items instead of states: an 8-lane slot loads an item's children once and accumulates
+// both parents, each in its own plan order, so every state's sum is the same bits as the
+// single-parent kernels'.  Parent p + 1's pair for partner j is parent p's plus K - p - 1
+// (pair_index(i, j) = (i-1)K - i(i+1)/2 + j).  Across N = 24..32 this removes 26% of the child
+// and plan-word loads.
+//
+// Item flags of one level: bit 0 = state r (0-based in the level) starts an item (p - p_lo even),
+// bit 1 = the next state is the second parent of that item (same T: its p is p + 1).
+__device__ inline int pair_row_of(int64_t q, int K, int plo) {   // q: 1-based pair index
+    int p = plo;
+    while (p + 1 < K && (int64_t)p * K - (int64_t)(p + 1) * (p + 2) / 2 + p + 2 <= q) p++;
+    return p;
+}
+
+template <typename W>
+__global__ __launch_bounds__(256) void k_item_flags(const W *__restrict__ trans_lvl, int64_t nst,
+                                                    int k, int K, int plo,
+                                                    uint8_t *__restrict__ flags) {
+    constexpr int bits = Packing<W>::bits;
+    constexpr W mask = (W)((W(1) << bits) - 1);
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nst) return;
+    const int p = pair_row_of((int64_t)(trans_lvl[r * k] & mask), K, plo);
+    const bool start = ((p - plo) & 1) == 0;
+    bool two = false;
+    if (start && r + 1 < nst)
+        two = pair_row_of((int64_t)(trans_lvl[(r + 1) * k] & mask), K, plo) == p + 1;
+    flags[r] = (uint8_t)((start ? 1 : 0) | (two ? 2 : 0));
+}
+
+template <int DT, typename W, int WPB, bool A32, int UB>
+__device__ inline void level_lds_vec2_body_a(const LevelArgs<DT, W> &a, int64_t j,
+                                             typename Elt<DT>::P *lds) {
+    using E = Elt<DT>;
+    using PT = typename E::P;
+    using HT = typename E::H;
+    constexpr int bits = Packing<W>::bits;
+    constexpr int VEC = 16 / (int)sizeof(HT);
+    constexpr int LPS = 8;                  // lanes per item
+    constexpr int TW = LPS * VEC;           // instances per tile
+    constexpr int SPW = 64 / LPS;           // items per wave
+    struct alignas(16) HV { HT v[VEC]; };
+    struct alignas(VEC * sizeof(PT)) PV { PT v[VEC]; };
+    const LdsMap &map = a.map;
+    const int lane = (int)(threadIdx.x & 63);
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int slot = lane / LPS, li = lane % LPS;
+    const int64_t xcd = j & 7, r = j >> 3;
+    int64_t tile, i_begin, i_end, bi, bstride;
+    if (map.tiles_xcd > 0) {
+        const int64_t ti = r % map.tiles_xcd;
+        tile = xcd + 8 * ti;
+        bi = r / map.tiles_xcd;
+        bstride = map.per_xcd / map.tiles_xcd;
+        i_begin = 0;
+        i_end = a.nitems;
+        if (tile >= map.ntiles) return;
+    } else {
+        tile = xcd % map.ntiles;
+        const int64_t group = xcd / map.ntiles;
+        const int64_t ngroups = (8 - tile + map.ntiles - 1) / map.ntiles;
+        i_begin = group * a.nitems / ngroups;
+        i_end = (group + 1) * a.nitems / ngroups;
+        bi = r;
+        bstride = map.per_xcd;
+    }
+    const int64_t b = tile * TW + li * VEC;   // this lane's first instance
+
+    // stage pair rows [qlo, qlo + nq) of this tile: row i at lds[i * TW]
+    for (int e = (int)threadIdx.x; e < a.nq * TW; e += WPB * 64) {
+        const int64_t bb = tile * TW + (e % TW);
+        lds[e] = bb < a.nb ? a.P[(int64_t)(a.qlo + e / TW) * a.ldp + bb] : PT{};
+    }
+    __syncthreads();
+
+    const char *hbase = reinterpret_cast<const char *>(a.Hc) - a.cfirst * a.ldh * (int64_t)sizeof(HT);
+    const uint32_t hrow = (uint32_t)(a.ldh * (int64_t)sizeof(HT));
+    const uint32_t hlane = (uint32_t)(b * (int64_t)sizeof(HT));
+    const char *lbase = reinterpret_cast<const char *>(lds) +
+                        ((int64_t)li * VEC - (int64_t)(1 + a.qlo) * TW) * (int64_t)sizeof(PT);
+    auto load_child = [&](W t) -> HV {
+        if constexpr (A32) {
+            const uint32_t off = __umul24((uint32_t)((uint64_t)t >> bits), hrow) + hlane;
+            return *reinterpret_cast<const HV *>(hbase + (uint64_t)off);
+        } else {
+            return *reinterpret_cast<const HV *>(a.Hc + ((int64_t)(t >> bits) - a.cfirst) * a.ldh + b);
+        }
+    };
+    auto pair_at = [&](uint32_t q) -> PV {
+        return *reinterpret_cast<const PV *>(lbase + q * (uint32_t)(TW * sizeof(PT)));
+    };
+
+    const int64_t step = bstride * WPB * SPW;
+    constexpr int U = UB;
+    constexpr int NW = sizeof(W) == 4 ? 4 : 8;   // k <= 31 (Int32 layout) or 63 (Int64)
+    for (int64_t i0 = i_begin + (bi * WPB + wave) * SPW; i0 < i_end; i0 += step) {
+        const int64_t it = i0 + slot;
+        const bool ok = it < i_end;
+        const uint32_t item = a.items[ok ? it : i_end - 1];
+        const int64_t s = (int64_t)(item & 0x7fffffffu);
+        const bool two = ok && (item >> 31) != 0;
+        const W *tw = a.trans_lvl + s * (int64_t)a.k;
+        W wv[NW];
+#pragma unroll
+        for (int c = 0; c < NW; c++)
+            if (c * LPS < a.k) wv[c] = tw[min(c * LPS + li, a.k - 1)];
+        auto word = [&](int t) -> W {   // t uniform
+            W w = 0;
+#pragma unroll
+            for (int c = 0; c < NW; c++)
+                if (t / LPS == c) w = (W)__shfl(wv[c], t % LPS, LPS);
+            return w;
+        };
+        // the second parent's pair offset: K - p - 1, p = min S of the first parent
+        const int p = pair_row_of((int64_t)__builtin_amdgcn_ubfe((uint32_t)word(0), 0, bits), a.K, a.plo);
+        const uint32_t d = (uint32_t)(a.K - p - 1);
+        HT acc0[VEC], acc1[VEC];
+#pragma unroll
+        for (int v = 0; v < VEC; v++) acc0[v] = acc1[v] = E::zero();
+        auto batch = [&](int i, auto n) {
+            constexpr int N = decltype(n)::value;
+            W t[N];
+            HV h[N];
+#pragma unroll
+            for (int u = 0; u < N; u++) t[u] = word(i + u);
+#pragma unroll
+            for (int u = 0; u < N; u++) h[u] = load_child(t[u]);
+#pragma unroll
+            for (int u = 0; u < N; u++) {
+                const uint32_t q = __builtin_amdgcn_ubfe((uint32_t)t[u], 0, bits);
+                const PV p0 = pair_at(q);
+#pragma unroll
+                for (int v = 0; v < VEC; v++) acc0[v] = E::muladd(acc0[v], p0.v[v], h[u].v[v]);
+                if (two) {
+                    const PV p1 = pair_at(q + d);
+#pragma unroll
+                    for (int v = 0; v < VEC; v++) acc1[v] = E::muladd(acc1[v], p1.v[v], h[u].v[v]);
+                }
+            }
+        };
+        int i = 0;
+        for (; i + U <= a.k; i += U) batch(i, std::integral_constant<int, U>{});
+        batch_tail<U>(a.k - i, [&](auto n) { batch(i, n); });
+        if (ok) {
+#pragma unroll
+            for (int v = 0; v < VEC; v++)
+                if (b + v < a.nb) a.Ho[s * a.ldo + b + v] = acc0[v];
+            if (two) {
+#pragma unroll
+                for (int v = 0; v < VEC; v++)
+                    if (b + v < a.nb) a.Ho[(s + 1) * a.ldo + b + v] = acc1[v];
+            }
+        }
+    }
+}
+
+template <int DT, typename W, int WPB, int UB = 4>
+__device__ inline void level_lds_vec2_body(const LevelArgs<DT, W> &a, int64_t j,
+                                           typename Elt<DT>::P *lds) {
+    if (a.addr32) level_lds_vec2_body_a<DT, W, WPB, true, UB>(a, j, lds);
+    else level_lds_vec2_body_a<DT, W, WPB, false, UB>(a, j, lds);
+}
+
+template <int DT, typename W, int WPB, int TAG, int UB = 4>
+__global__ __launch_bounds__(WPB * 64) void k_level_fwd_lds_vec2(LevelArgs<DT, W> a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    level_lds_vec2_body<DT, W, WPB, UB>(a, (int64_t)blockIdx.x,
+                                        reinterpret_cast<typename Elt<DT>::P *>(lds_raw));
+}
+
 // UB: transitions per batch of child loads, 4 or 6 (the autotuner picks; 6 pays with passes of
 // 64 instances at N=28)
 template <int DT, typename W, int WPB, int TAG, int UB = 4>
@@ -710,7 +890,7 @@ __global__ __launch_bounds__(256) void k_gather_pairs(const PT *__restrict__ A, 
 // out[b] = log2 of min((K-1)!! * m^(K/2), sqrt(prod_i max(r_i, 1))), m = max(1, max |p|) and
 // r_i = sum_{j != i} |p_ij|, an upper bound on |haf| and on every (K-2)-minor of instance b
 // (for entrywise absolute values haf^2 <= perm <= the product of the row sums).
-__global__ __launch_bounds__(256) void k_exact_log2_bound(const long long *__restrict__ P,
+static __global__ __launch_bounds__(256) void k_exact_log2_bound(const long long *__restrict__ P,
                                                           int64_t ldp, int64_t B, int K,
                                                           double *__restrict__ out) {
     const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

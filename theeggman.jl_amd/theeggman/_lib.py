@@ -51,7 +51,7 @@ SIGNATURES = {
     "eggman_dp_forward_device": (_int, [ctypes.c_uint64, _int, _vp, _i64, _i64, _vp, _vp]),
     "eggman_dp_forward_device_noalloc": (_int, [ctypes.c_uint64, _int, _vp, _i64, _i64, _vp, _vp]),
     "eggman_dp_forward_repeated": (_int, [ctypes.c_uint64, _int, _vp, _i64, _vp, _i64, _i64, _vp]),
-    "eggman_dp_forward_mixed": (_int, [_int, _u64p, _i64p, _int, _vp, _vp]),
+    "eggman_dp_forward_mixed": (_int, [_int, _u64p, _i64p, _int, _vp, _i64, _vp]),
     "eggman_dp_forward_mixed_device": (_int, [_int, _u64p, _i64p, _int, _vp, _vp, _vp]),
     "eggman_dp_forward_grad": (_int, [ctypes.c_uint64, _int, _vp, _i64, _i64, _i64, _vp, _vp]),
     "eggman_dp_forward_grad_device": (_int, [ctypes.c_uint64, _int, _vp, _i64, _i64, _vp, _vp,

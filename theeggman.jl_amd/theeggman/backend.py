@@ -177,10 +177,13 @@ def _haf_dp_backend_repeated(backend: HIPBackend, A: np.ndarray, idxs, plan, *,
     return out
 
 
-def _haf_dp_backend_mixed(backend: HIPBackend, Ps, plans):
+def _haf_dp_backend_mixed(backend: HIPBackend, Ps, plans, *, max_batch: int | None = None):
     """Hafnians of several groups of instances, group g of degree plans[g].K with pair values
-    Ps[g] (shape (B_g, npairs_g)), in one device pass that advances every group by one level per
-    launch.  Returns one array (or list of ints, exact path) per group.  New entry point."""
+    Ps[g] (shape (B_g, npairs_g)), in one device pass: groups of 256 or more instances walk
+    concurrently on streams of their own, the smaller ones share a walk that advances every group
+    by one level per launch.  ``max_batch`` bounds the instances of any one group on the device
+    at once (chunks take the same share of every group; the values are the same bits).  Returns one array (or
+    list of ints, exact path) per group.  New entry point."""
     if not isinstance(backend, HIPBackend):
         raise ArgumentError(f"no GPU backend method for {type(backend).__name__}")
     if len(Ps) != len(plans) or not Ps:
@@ -204,7 +207,8 @@ def _haf_dp_backend_mixed(backend: HIPBackend, Ps, plans):
     out = np.empty((total, 2), np.uint64) if code == I128 else np.empty(total, dt)
     h = (ctypes.c_uint64 * len(handles))(*handles)
     c = (ctypes.c_int64 * len(counts))(*counts)
-    check(lib().eggman_dp_forward_mixed(len(handles), h, c, code, flat.ctypes.data,
+    mb = 0 if max_batch is None else max(1, int(max_batch))
+    check(lib().eggman_dp_forward_mixed(len(handles), h, c, code, flat.ctypes.data, mb,
                                         out.ctypes.data), "dp_forward_mixed")
     res, o = [], 0
     for B in counts:

@@ -321,14 +321,15 @@ def hafnian_exact(A, *, check_symmetric: bool = True, backend=None,
     return vals if batch else vals[0]
 
 
-def hafnian_mixed(As, *, check_symmetric: bool = True, backend=None):
+def hafnian_mixed(As, *, check_symmetric: bool = True, backend=None, max_batch: int | None = None):
     """Hafnians of matrices of different sizes, in input order, in one device pass.
 
     New entry point: the reference's batch form requires one size (src/hafnian.jl:1277-1280).
     Matrices are grouped by degree and every group shares its plan.  Groups of 256 or more
-    matrices walk one after another with their own tuned launch; the smaller groups share one
-    walk in which each launch advances all of them by a level.  Each value is bit-identical to
-    ``hafnian`` of that matrix alone.
+    matrices walk concurrently, each with its own tuned launch on a stream of its own; the
+    smaller groups share one walk in which each launch advances all of them by a level.
+    ``max_batch`` bounds the matrices of any one degree on the device at once.  Each value is
+    bit-identical to ``hafnian`` of that matrix alone.
     Degrees above DP_MAX (Int64 layout) and up to it (Int32) run as two passes.
     """
     As = [_as_matrix(A) for A in As]
@@ -352,7 +353,7 @@ def hafnian_mixed(As, *, check_symmetric: bool = True, backend=None):
         if not Ns:
             continue
         Ps = [_gather_pairs_batch(T, np.stack([As[i] for i in groups[N]]), None, N) for N in Ns]
-        res = _haf_dp_backend_mixed(be, Ps, [_plan_for(N, backend) for N in Ns])
+        res = _haf_dp_backend_mixed(be, Ps, [_plan_for(N, backend) for N in Ns], max_batch=max_batch)
         for N, r in zip(Ns, res):
             out[groups[N]] = r
     return out

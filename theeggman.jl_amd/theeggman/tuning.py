@@ -21,9 +21,9 @@ from ._lib import check, lib
 from .backend import HIPBackend, _device_plan, dtype_code
 from .plan import DP_MAX, PlanRef, _dp_plan
 
-LDS, GRAPH, FUSED, WIDE, CKPT, NARROW, VEC, DEEP = 1, 2, 4, 8, 16, 32, 64, 128   # EGGMAN_TUNE_*
+LDS, GRAPH, FUSED, WIDE, CKPT, NARROW, VEC, DEEP, PAIRS = 1, 2, 4, 8, 16, 32, 64, 128, 256
 _NAMES = {"lds": LDS, "graph": GRAPH, "fused": FUSED, "wide": WIDE, "ckpt": CKPT, "narrow": NARROW,
-          "vec": VEC, "deep": DEEP}
+          "vec": VEC, "deep": DEEP, "pairs": PAIRS}   # EGGMAN_TUNE_*
 DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_mi355x.json")
 _TUNED: dict = {}   # (K, index_bytes, dtype code, B) -> {"bt", "chunk", "flags", "occ"}
 
@@ -72,14 +72,14 @@ def tune_get(K: int, dtype=np.float64, B: int = 1024):
 def tune_set(K: int, dtype, B: int, bt: int = 64, chunk: int = 0, occ: int = 0, *,
              lds: bool = True, graph: bool = False, fused: bool = False, wide: bool = False,
              ckpt: bool = False, narrow: bool = False, vec: bool = False, deep: bool = False,
-             index_bytes: int | None = None) -> None:
+             pairs: bool = False, index_bytes: int | None = None) -> None:
     """Set the tuning entry for (K, dtype, B) in the plan layout of ``index_bytes`` (default: the
     layout degree K uses, Int32 up to K = 32)."""
     ib = index_bytes or (4 if K <= DP_MAX else 8)
     code = dtype_code(dtype)
     flags = ((LDS if lds else 0) | (GRAPH if graph else 0) | (FUSED if fused else 0) |
              (WIDE if wide else 0) | (CKPT if ckpt else 0) | (NARROW if narrow else 0) |
-             (VEC if vec else 0) | (DEEP if deep else 0))
+             (VEC if vec else 0) | (DEEP if deep else 0) | (PAIRS if pairs else 0))
     check(lib().eggman_tune_set(K, ib, code, B, bt, chunk, flags, occ), "tune_set")
     _TUNED[(K, ib, code, B)] = {"bt": bt, "chunk": chunk, "flags": flags, "occ": occ}
 
