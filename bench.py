@@ -332,12 +332,110 @@ def run_configs(path):
     print(json.dumps(res), flush=True)
 
 
+def run_plans(path):
+    """Plan costs and device memory (the reference's GPU validation stages C4 and D,
+    benchmark/gpu/run_gpu_validation.jl:315-366, and its published plan build times,
+    src/dp.jl:54-55, 73-78): host build time of the library's plan builder, upload and
+    device-generation times, the calls it takes to amortise an upload, and the device bytes
+    a batch needs against the reference's dp_batch_bytes (src/dp.jl:584-587)."""
+    import numpy as np
+    import theeggman as te
+    from theeggman import _lib, hip, synth
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    L = _lib.lib()
+    st = hip.Stream()
+    res = {"plan": {}, "amortise": {}, "memory": {}}
+    for K in (20, 24, 28, 32, 36, 40):
+        ib = 4 if K <= 32 else 8
+        row = {"index_bytes": ib}
+        ns, nt = te._dp_counts(K)
+        row["bytes"] = (ns + nt + 1) * ib
+        if K <= 36:
+            t0 = time.perf_counter()
+            plan = te._build_dp_plan(K, np.int32 if ib == 4 else np.int64)
+            row["host_build_s"] = time.perf_counter() - t0
+        if K <= 32:
+            t0 = time.perf_counter()
+            oracle.build_plan(K)
+            row["oracle_build_s"] = time.perf_counter() - t0
+            oracle._PLANS.clear()
+            te.empty_gpu_cache()
+            t0 = time.perf_counter()
+            te._device_plan(te.HIPBackend(), plan)
+            hip.device_synchronize()
+            row["upload_s"] = time.perf_counter() - t0
+        te.empty_gpu_cache()
+        t0 = time.perf_counter()
+        te._device_plan(te.HIPBackend(plan_source="generate"), te.PlanRef(K, ib))
+        hip.device_synchronize()
+        row["generate_s"] = time.perf_counter() - t0
+        if K <= 36:
+            del plan
+        res["plan"][K] = row
+        log(f"plan K={K}: {row}")
+        te.empty_gpu_cache()
+    # upload amortisation: the first plain call of a degree (plan upload included) against the
+    # steady-state call, as stage C4 measures it
+    for K in (24, 28, 32):
+        plan = te._dp_plan(K)
+        A = synth.sym_complex_normal(K, 2, 0)
+        te.empty_gpu_cache()
+        t0 = time.perf_counter()
+        te.hafnian(A, backend=te.HIPBackend())
+        first = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for _ in range(20):
+            te.hafnian(A, backend=te.HIPBackend())
+        steady = (time.perf_counter() - t0) / 20
+        res["amortise"][K] = {"first_call_s": first, "steady_call_s": steady,
+                              "calls_to_amortise": (first - steady) / steady}
+        log(f"amortise K={K}: {res['amortise'][K]}")
+    # device memory one call needs beside the plan: forward (two levels per pass) and forward +
+    # gradient (the kept levels of a pass), measured as the library's cache bytes after a call
+    for K in (24, 28, 32):
+        B = 1024
+        plan = te._dp_plan(K)
+        te.empty_gpu_cache()
+        h = te._device_plan(te.HIPBackend(), plan)
+        base = L.eggman_cache_bytes()
+        As = np.stack([synth.sym_normal(K, 3, b, 1 / np.sqrt(K)) for b in range(B)])
+        P = synth.pair_batch(As)
+        Pd = hip.DeviceBuffer.from_array(P)
+        out = hip.DeviceBuffer(B * 8)
+        te._haf_dp_device(te.HIPBackend(), plan, np.float64, Pd.addr, B, B, out.addr, st.handle)
+        st.synchronize()
+        fwd = L.eggman_cache_bytes() - base
+        dP = hip.DeviceBuffer(B * 8 * P.shape[0])
+        _lib.check(L.eggman_dp_forward_grad_device(h, _lib.F64, Pd.addr, B, B, out.addr, dP.addr, B,
+                                                   st.handle))
+        st.synchronize()
+        grad = L.eggman_cache_bytes() - base
+        ns, _ = te._dp_counts(K)
+        npairs = K * (K - 1) // 2
+        res["memory"][K] = {
+            "batch": B, "reference_dp_batch_bytes": int(te.dp_batch_bytes(K, np.float64, B)),
+            "forward_work_bytes": int(fwd), "forward_plus_gradient_work_bytes": int(grad),
+            "caller_bytes_per_instance_forward": (npairs + 1) * 8,
+            "caller_bytes_per_instance_gradient": (2 * npairs + 1) * 8,
+            "note": "forward work memory is two levels of one pass (independent of B); the "
+                    "reference holds every state of every instance (dp_batch_bytes)"}
+        log(f"memory K={K}: {res['memory'][K]}")
+        del Pd, out, dP
+    with open(path, "w") as f:
+        json.dump(res, f, indent=1, default=str)
+    print(json.dumps(res, default=str), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--configs", metavar="PATH",
                     help="instead of the benchmark line, measure the five workload configurations "
                          "and write them to PATH")
+    ap.add_argument("--plans", metavar="PATH",
+                    help="instead of the benchmark line, measure plan build/upload/generation "
+                         "times, upload amortisation and device memory, and write them to PATH")
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=1024)
@@ -360,6 +458,9 @@ def main():
         sys.exit(2)
     if args.configs:
         run_configs(args.configs)
+        return
+    if args.plans:
+        run_plans(args.plans)
         return
 
     import numpy as np

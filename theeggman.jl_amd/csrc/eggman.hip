@@ -801,6 +801,7 @@ static bool launch_level_lds_w(const DevCaps &caps, const Tune &tune, const DevP
     a.nitems = KIND == KIND_VEC2 ? pl.nitems[L - 1] : 0;
     a.K = lay.K;
     a.plo = lay.lo[L - 1];
+    a.vstore = ((uintptr_t)Ho & 15) == 0 && ldo % (16 / (int64_t)sizeof(typename E::H)) == 0;
     kern<<<dim3((unsigned)(8 * map.per_xcd)), dim3(WPB * 64), lds, st>>>(a);
     return true;
 }
@@ -1385,6 +1386,11 @@ static int forward_mixed_levels(DevState &ds, const std::vector<DevPlan *> &pls,
                 a.map = make_lds_map(ds.caps, tg[g].occ, a.nst, nb, lds, WPB, TW,
                                      KIND == KIND_VEC ? 8 : KIND == KIND_NARROW ? 4 : 1);
                 a.addr32 = child_addr32(lay.nstates, ld * (int64_t)sizeof(HT));
+                a.items = nullptr;
+                a.nitems = 0;
+                a.K = lay.K;
+                a.plo = lay.lo[L - 1];
+                a.vstore = ((uintptr_t)Ho & 15) == 0 && ld % (16 / (int64_t)sizeof(HT)) == 0;
                 m.first[m.n + 1] = m.first[m.n] + 8 * a.map.per_xcd;
                 m.n++;
                 lds_max = std::max(lds_max, lds);

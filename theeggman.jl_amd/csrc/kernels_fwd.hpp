@@ -181,6 +181,9 @@ struct LevelArgs {
     const uint32_t *items;
     int64_t nitems;
     int K, plo;
+    // the 16-byte-load kernels store a lane's instances as one 16-byte store: Ho is 16-byte
+    // aligned and ldo a multiple of 16 / sizeof(H) (checked by the launcher)
+    bool vstore;
 };
 
 // Launch-time check behind LevelArgs::addr32 and the other kernels' 32-bit address paths: a
@@ -502,9 +505,16 @@ __device__ inline void level_lds_vec_body_a(const LevelArgs<DT, W> &a, int64_t j
         // the last k mod U transitions as one batch too (k is uniform, so is the branch)
         batch_tail<U>(a.k - i, [&](auto n) { batch(i, n); });
         if (ok) {
+            if (a.vstore && b + VEC <= a.nb) {
+                HV o;
 #pragma unroll
-            for (int v = 0; v < VEC; v++)
-                if (b + v < a.nb) a.Ho[s * a.ldo + b + v] = acc[v];
+                for (int v = 0; v < VEC; v++) o.v[v] = acc[v];
+                *reinterpret_cast<HV *>(a.Ho + s * a.ldo + b) = o;
+            } else {
+#pragma unroll
+                for (int v = 0; v < VEC; v++)
+                    if (b + v < a.nb) a.Ho[s * a.ldo + b + v] = acc[v];
+            }
         }
     }
 }
@@ -665,13 +675,24 @@ __device__ inline void level_lds_vec2_body_a(const LevelArgs<DT, W> &a, int64_t 
         for (; i + U <= a.k; i += U) batch(i, std::integral_constant<int, U>{});
         batch_tail<U>(a.k - i, [&](auto n) { batch(i, n); });
         if (ok) {
+            if (a.vstore && b + VEC <= a.nb) {
+                HV o0, o1;
 #pragma unroll
-            for (int v = 0; v < VEC; v++)
-                if (b + v < a.nb) a.Ho[s * a.ldo + b + v] = acc0[v];
-            if (two) {
+                for (int v = 0; v < VEC; v++) {
+                    o0.v[v] = acc0[v];
+                    o1.v[v] = acc1[v];
+                }
+                *reinterpret_cast<HV *>(a.Ho + s * a.ldo + b) = o0;
+                if (two) *reinterpret_cast<HV *>(a.Ho + (s + 1) * a.ldo + b) = o1;
+            } else {
 #pragma unroll
                 for (int v = 0; v < VEC; v++)
-                    if (b + v < a.nb) a.Ho[(s + 1) * a.ldo + b + v] = acc1[v];
+                    if (b + v < a.nb) a.Ho[s * a.ldo + b + v] = acc0[v];
+                if (two) {
+#pragma unroll
+                    for (int v = 0; v < VEC; v++)
+                        if (b + v < a.nb) a.Ho[(s + 1) * a.ldo + b + v] = acc1[v];
+                }
             }
         }
     }
