@@ -125,7 +125,7 @@ def traffic_from_profiles(kernel, chunk):
             prof = json.load(f)
     except (OSError, ValueError):
         return None
-    if prof.get("kernel", "").split("<")[0] != kernel or prof.get("chunk", chunk) != chunk:
+    if prof.get("kernel", "").split("<")[0].split("::")[-1] != kernel or prof.get("chunk", chunk) != chunk:
         return None
     return prof
 
