@@ -564,6 +564,26 @@ def main():
         del csrc, cdst
         rl["measured_copy_gbs"] = copy_gbs
         rl["frac_of_copy"] = rl["achieved"] / copy_gbs
+        # the practical ceiling of the level kernel's gathers: random reads of whole 128-byte
+        # lines (8 lanes x 16 bytes, as the kernel reads a child row) from a working set the size
+        # of a pass's two level buffers, measured on this GPU
+        import ctypes
+        probe = {}
+        for name, ws in (("l2_resident", 2 << 20), ("pass_buffers", 2 * int(nst.max()) * chunk * 8)):
+            lps = ctypes.c_double()
+            _lib.check(_lib.lib().eggman_gather_rate(ws, ctypes.byref(lps)), "gather_rate")
+            probe[name] = {"working_set_bytes": ws, "line_gbs": lps.value * 128 / 1e9}
+        rl["gather_probe"] = probe
+        if prof and prof.get("l1_to_l2_read_requests_per_launch"):
+            # the kernel's L1-to-L2 line requests per second (counter run) against the probe's
+            # rates blended by the kernel's L2 hit rate: its hits at the L2-resident rate, its
+            # misses at the rate of a working set the size of a pass's two level buffers
+            kl = prof["l1_to_l2_read_requests_per_launch"] * 128 / (rl["avg_launch_ms"] / 1e3) / 1e9
+            h = prof["l2"]["hit_rate"]
+            ceil = 1.0 / (h / probe["l2_resident"]["line_gbs"] + (1 - h) / probe["pass_buffers"]["line_gbs"])
+            rl["l1_to_l2_line_gbs"] = kl
+            rl["gather_ceiling_gbs"] = ceil
+            rl["frac_of_gather_ceiling"] = kl / ceil
     if rl["traffic"]:
         rl["traffic_gbs"] = rl["traffic"] / (rl["avg_launch_ms"] / 1e3) / 1e9
         rl["traffic_frac"] = rl["traffic_gbs"] / HBM_PEAK_GBS

@@ -265,6 +265,16 @@ int eggman_tune_get(int K, int index_bytes, int dtype, int64_t B, int *bt, int64
  */
 int eggman_exact_bound(int K, const int64_t *P, int64_t ldp, int64_t B, double *log2_bound);
 
+/*
+ * Measurement helper (no counterpart in the reference): the rate, in 128-byte lines per second,
+ * at which the current device serves random reads of whole lines -- each read the way the
+ * 16-byte-load level kernels read a child row, 8 lanes x 16 bytes -- from a zeroed working set of
+ * working_set_bytes (sized like a pass's level buffers, so served by L2, MALL or HBM as the
+ * walk's child reads are).  The practical ceiling of the level kernels' gathers, reported by the
+ * benchmark beside the device-copy rate.  Allocates and frees its working set.
+ */
+int eggman_gather_rate(int64_t working_set_bytes, double *lines_per_s);
+
 /* ---------------------------------------------------------------- caches */
 
 /* Device bytes held by cached plans and work buffers: the library's share of gpu_cache_bytes

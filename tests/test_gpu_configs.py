@@ -151,3 +151,17 @@ def test_exact_path_refuses_out_of_range_instances():
     hip.device_synchronize()
     lo, hi = out.download(np.uint64, 2)
     assert te.backend._limbs_to_int(int(lo), int(hi)) == math.prod(range(39, 0, -2)) * 5 ** 20
+
+
+def test_gather_probe_reports_line_rates():
+    # the measurement helper behind the benchmark's gather ceiling: a line rate for an L2-sized
+    # and a MALL-sized working set, the smaller one at least as fast
+    import ctypes
+    rates = []
+    for ws in (2 << 20, 128 << 20):
+        r = ctypes.c_double()
+        _lib.check(_lib.lib().eggman_gather_rate(ws, ctypes.byref(r)), "gather_rate")
+        rates.append(r.value * 128)
+        note(f"  gather probe {ws >> 20} MiB: {r.value * 128 / 1e12:.2f} TB/s")
+    assert 1e11 < rates[1] <= rates[0] * 1.05
+    assert _lib.lib().eggman_gather_rate(64, ctypes.byref(ctypes.c_double())) == _lib.EINVAL

@@ -2786,6 +2786,48 @@ int eggman_exact_bound(int K, const int64_t *P, int64_t ldp, int64_t B, double *
     return rc;
 }
 
+int eggman_gather_rate(int64_t working_set_bytes, double *lines_per_s) {
+    std::lock_guard<std::recursive_mutex> lk(g_mu);
+    if (working_set_bytes < 128 || !lines_per_s)
+        return fail(EGGMAN_EINVAL, "need a working set of at least one 128-byte line");
+    int dev;
+    EGG_TRY(current_device(&dev));
+    DevState *ds;
+    EGG_TRY(dev_state(dev, &ds));
+    EGG_TRY(chain_stream(*ds, ds->stream));
+    const uint64_t nlines = (uint64_t)working_set_bytes / 128;
+    DevBuf buf, sink;
+    EGG_TRY(ensure(buf, (size_t)nlines * 128));
+    EGG_TRY(ensure(sink, 64));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = EGGMAN_OK;
+    float ms = 0;
+    const int blocks = ds->caps.cus * 4, iters = 64, reps = 5;
+    // zero the working set, then one warm-up launch
+    if (hipMemsetAsync(buf.p, 0, buf.bytes, ds->stream) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = fail(EGGMAN_EHIP, "gather probe setup failed");
+    } else {
+        k_gather_probe<<<dim3((unsigned)blocks), dim3(512), 0, ds->stream>>>(
+            (const double2 *)buf.p, nlines, iters, 1ull, (double *)sink.p);
+        (void)hipEventRecord(e0, ds->stream);
+        for (int r = 0; r < reps; r++)
+            k_gather_probe<<<dim3((unsigned)blocks), dim3(512), 0, ds->stream>>>(
+                (const double2 *)buf.p, nlines, iters, (uint64_t)r + 2, (double *)sink.p);
+        (void)hipEventRecord(e1, ds->stream);
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            rc = fail(EGGMAN_EHIP, "gather probe timing failed");
+    }
+    (void)hipStreamSynchronize(ds->stream);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(buf.p);
+    (void)hipFree(sink.p);
+    if (rc != EGGMAN_OK) return rc;
+    *lines_per_s = (double)reps * blocks * 64.0 * 8.0 * iters / (ms / 1e3);
+    return EGGMAN_OK;
+}
+
 int64_t eggman_cache_bytes(void) {
     std::lock_guard<std::recursive_mutex> lk(g_mu);
     int64_t total = 0;

@@ -937,6 +937,32 @@ static __global__ __launch_bounds__(256) void k_exact_log2_bound(const long long
     out[b] = fmin(lf, 0.5 * lr);
 }
 
+// Measurement helper (no counterpart in the reference): the rate at which the device serves
+// random reads of whole 128-byte lines, each read the way the 16-byte-load level kernels read a
+// child row (8 lanes x 16 bytes), from a working set of `nlines` lines.  Every line index is
+// reduced modulo nlines, so every read is in bounds.  The practical ceiling of the level
+// kernels' gathers, next to the device-copy rate.
+static __global__ __launch_bounds__(512) void k_gather_probe(const double2 *__restrict__ buf,
+                                                             uint64_t nlines, int iters, uint64_t seed,
+                                                             double *__restrict__ sink) {
+    const int li = (int)(threadIdx.x & 7);
+    uint64_t x = seed + (uint64_t)(blockIdx.x * 64 + (threadIdx.x >> 3)) * 0x9E3779B97F4A7C15ull;
+    double acc = 0.0;
+    for (int it = 0; it < iters; it++) {
+        double2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            x ^= x >> 31;
+            x *= 0xBF58476D1CE4E5B9ull;
+            x ^= x >> 29;
+            v[u] = buf[(x % nlines) * 8 + li];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc += v[u].x + v[u].y;
+    }
+    if (acc == 1.25e300) sink[0] = acc;   // never true for the zeroed buffer; keeps the loads
+}
+
 // Row of ones for the empty set (H[1] = one, src/dp.jl:421): a plain store kernel, so the
 // sequence stays valid inside a captured graph.
 template <int DT>

@@ -61,6 +61,7 @@ SIGNATURES = {
     "eggman_tune_get": (_int, [_int, _int, _int, _i64, ctypes.POINTER(_int), _i64p,
                                ctypes.POINTER(_int), ctypes.POINTER(_int)]),
     "eggman_exact_bound": (_int, [_int, _vp, _i64, _i64, _vp]),
+    "eggman_gather_rate": (_int, [_i64, ctypes.POINTER(ctypes.c_double)]),
     "eggman_cache_bytes": (_i64, []),
     "eggman_cache_clear": (_int, []),
     "eggman_batch_bytes": (_i64, [_int, _int, _i64]),
