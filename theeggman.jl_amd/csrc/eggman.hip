@@ -68,6 +68,13 @@ struct DevPlan {
     void *d_red = nullptr, *d_srow0 = nullptr, *d_sruns = nullptr;
     std::vector<int64_t> lvl_task;
     int64_t part_rows = 0, nred = 0;
+    // fused child-major reverse step (plan_host.hpp: FusePlanHost), Int32 layout
+    bool fuse = false;
+    void *d_fent = nullptr, *d_fco = nullptr, *d_cmask = nullptr, *d_fprow0 = nullptr;
+    void *d_fterms = nullptr, *d_ftoff = nullptr, *d_ftasks = nullptr, *d_frtask = nullptr, *d_fred = nullptr;
+    void *d_fsrow0 = nullptr, *d_fsruns = nullptr;
+    std::vector<FuseLevel> flv;
+    int64_t fpart_rows = 0, fnred = 0;
     size_t rev_bytes = 0;
     GradLevels gl_full, gl_ckpt;   // where the gradient keeps forward levels (plan_host.hpp)
     // parent-pair items of each level (k_level_fwd_lds_vec2), built on first use: level L's
@@ -135,6 +142,7 @@ struct DevState {
     DevBuf A;                        // staged matrix (repeated-pattern batches)
     DevBuf bnd;                      // per-instance range bounds of exact device-pointer calls
     DevBuf Hm;                       // the walks of a mixed call's large groups, one slice each
+    DevBuf ftot;                     // small-batch stripe totals of the fused reverse step
     // side streams of mixed calls (default priority), forked from and joined to the call's stream
     static constexpr int kSide = 4;
     hipStream_t side[kSide] = {};
@@ -148,7 +156,7 @@ struct DevState {
     DevBuf din[2], dout[2];
     size_t bytes() const {
         return H.bytes + Hw.bytes + Hg.bytes + G.bytes + part.bytes + A.bytes + bnd.bytes + Hm.bytes +
-               din[0].bytes + din[1].bytes + dout[0].bytes + dout[1].bytes;
+               ftot.bytes + din[0].bytes + din[1].bytes + dout[0].bytes + dout[1].bytes;
     }
 };
 
@@ -1629,6 +1637,33 @@ static int ensure_rev(DevPlan &pl) {
     pl.nred = (int64_t)rp.red.size();
     pl.lvl_seg = rp.lvl_seg;
     pl.nseg = (int64_t)rp.segs.size();
+    if (lay.index_bytes == 4) {
+        FusePlanHost fp;
+        EGG_TRY(build_fuse_plan(lay, rp, kRedFanIn, &fp));
+        std::vector<ParTaskDev> ft(fp.tasks.size());
+        for (size_t i = 0; i < ft.size(); i++) {
+            const ParTask &t = fp.tasks[i];
+            ft[i] = ParTaskDev{t.start, t.row0, t.len, t.p, t.level, 0};
+        }
+        std::vector<RedRowDev> fr(fp.red.size());
+        for (size_t i = 0; i < fr.size(); i++)
+            fr[i] = RedRowDev{fp.red[i].tb, fp.red[i].te, fp.red[i].u, 0};
+        EGG_TRY(upload_vec(&pl.d_fent, fp.fent.data(), fp.fent.size() * sizeof(int32_t), &acc));
+        EGG_TRY(upload_vec(&pl.d_fco, fp.fco.data(), fp.fco.size() * sizeof(int32_t), &acc));
+        EGG_TRY(upload_vec(&pl.d_cmask, fp.cmask.data(), fp.cmask.size() * sizeof(uint32_t), &acc));
+        EGG_TRY(upload_vec(&pl.d_fprow0, fp.prow0.data(), fp.prow0.size() * sizeof(int64_t), &acc));
+        EGG_TRY(upload_vec(&pl.d_fterms, fp.terms.data(), fp.terms.size() * sizeof(int32_t), &acc));
+        EGG_TRY(upload_vec(&pl.d_ftoff, fp.toff.data(), fp.toff.size() * sizeof(int64_t), &acc));
+        EGG_TRY(upload_vec(&pl.d_ftasks, ft.data(), ft.size() * sizeof(ParTaskDev), &acc));
+        EGG_TRY(upload_vec(&pl.d_frtask, fp.rtask.data(), fp.rtask.size() * sizeof(int32_t), &acc));
+        EGG_TRY(upload_vec(&pl.d_fred, fr.data(), fr.size() * sizeof(RedRowDev), &acc));
+        EGG_TRY(upload_vec(&pl.d_fsrow0, fp.srow0.data(), fp.srow0.size() * sizeof(int32_t), &acc));
+        EGG_TRY(upload_vec(&pl.d_fsruns, fp.sruns.data(), fp.sruns.size() * sizeof(int32_t), &acc));
+        pl.flv = fp.lv;
+        pl.fpart_rows = fp.part_rows;
+        pl.fnred = (int64_t)fp.red.size();
+        pl.fuse = true;
+    }
     grad_levels_full(lay, &pl.gl_full);
     grad_levels_checkpointed(lay, &pl.gl_ckpt);
     pl.rev_bytes = acc;
@@ -1639,13 +1674,18 @@ static int ensure_rev(DevPlan &pl) {
 // Device bytes one instance needs for forward + reverse: the kept levels of H and the segment
 // region (every level, without checkpoints), two levels of G, the dP partials (segment sums, or
 // task rows plus stage-one rows), and its P and dP columns.
+// The fused child-major reverse step serves the Int32 layout and elements of at most 8 bytes
+// (its per-wave totals live in registers); the others keep the parent-major or segment dP.
+static bool use_fused(const DevPlan &pl, int dtype) { return pl.fuse && elt_hbytes(dtype) <= 8; }
+
 static size_t grad_bytes_per_instance(const DevPlan &pl, int dtype, bool ckpt) {
     const size_t hb = elt_hbytes(dtype), pb = elt_pbytes(dtype);
     const PlanLayout &lay = pl.lay;
     const GradLevels &gl = ckpt ? pl.gl_ckpt : pl.gl_full;
+    const int64_t part = use_fused(pl, dtype) ? pl.fpart_rows + pl.fnred
+                                              : std::max<int64_t>(pl.nseg, pl.part_rows + pl.nred);
     return (size_t)(gl.kept_rows + gl.seg_rows + 2 * lay.max_level) * hb +
-           (size_t)std::max<int64_t>(std::max<int64_t>(pl.nseg, pl.part_rows + pl.nred), 1) * hb +
-           (size_t)lay.npairs * (pb + hb);
+           (size_t)std::max<int64_t>(part, 1) * hb + (size_t)lay.npairs * (pb + hb);
 }
 
 // Waves per workgroup of the parent-major dP kernel at level L: the most (of 8, 4, 2) whose
@@ -1756,10 +1796,12 @@ static void launch_dp_par(const DevCaps &caps, const DevPlan &pl, int L, const v
     }
 }
 
+// `lists`: the child-major parent entries the sums follow (pl.d_cent, parents ascending, or the
+// fused step's pl.d_fent, pairs ascending); both share the offsets pl.d_coff.
 template <int DT, typename W>
 static void launch_rev_g(const DevCaps &caps, const Tune &tune, const DevPlan &pl, int L,
                          const void *P, int64_t ldp, const void *Gs, void *Gc, int64_t ldg,
-                         int64_t nb, hipStream_t st) {
+                         int64_t nb, hipStream_t st, const void *lists) {
     using E = Elt<DT>;
     const PlanLayout &lay = pl.lay;
     int qlo, nq;
@@ -1767,7 +1809,7 @@ static void launch_rev_g(const DevCaps &caps, const Tune &tune, const DevPlan &p
     const int64_t nch = lay.nst[L - 2];
     const size_t lds = (size_t)nq * 64 * sizeof(typename E::P);
     const bool use_lds = tune.lds && lds <= caps.max_lds_block;
-    const W *cent = (const W *)pl.d_cent;
+    const W *cent = (const W *)lists;
     const int64_t *coff = (const int64_t *)pl.d_coff + (lay.levels[L - 2] - 1);
     if (pow2_at_least(nb) < 64) {
         // batches of at most 32: as many lanes per child as instances (next power of two)
@@ -1810,6 +1852,59 @@ static void launch_rev_g(const DevCaps &caps, const Tune &tune, const DevPlan &p
         (typename E::H *)Gc, nb, map);
 }
 
+// Stripe totals of the small-batch fused step: one per (phase, wave-stripe, slot, instance).
+static size_t fuse_tot_bytes(const FuseLevel &f, int dtype) {
+    return (size_t)(f.phi - f.plo + 1) * f.S * kFuseWaves * f.nslot * elt_hbytes(dtype);
+}
+
+// The reverse step of level L by the fused order: G of level L-1 into Gc and the level's dP
+// partial rows.  64 instances or more take k_rev_fused; fewer take the adjoint gather over the
+// sorted lists and the term-list and stripe-sum kernels (totals in ds.ftot, sized by the caller).
+template <int DT, typename W>
+static void launch_rev_fused(DevState &ds, const DevPlan &pl, const Tune &tune, int L, const void *P,
+                             int64_t ldp, const void *Hc, int64_t ldh, const void *Gs, void *Gc,
+                             int64_t ldg, void *part, int64_t nb, hipStream_t st) {
+    using E = Elt<DT>;
+    using HT = typename E::H;
+    using PT = typename E::P;
+    const FuseLevel &f = pl.flv[L];
+    const int nph = f.phi - f.plo + 1;
+    if (nb >= 64) {
+        const int64_t ntiles = (nb + 63) / 64;
+        const size_t lds = (size_t)f.nslot * 64 * (sizeof(PT) + kFuseWaves * sizeof(HT));
+        const int64_t grid = ntiles >= 8 ? 8 * (int64_t)f.S * ((ntiles + 7) / 8)
+                                         : 8 * (((int64_t)f.S + 8 / ntiles - 1) / (8 / ntiles));
+        // 32-bit row offsets when every level row this step touches lies within 4 GB of its base
+        const int64_t rows_max = std::max<int64_t>(pl.lay.nst[L - 1], f.nch);
+        const bool a32 = rows_max * std::max(ldh, ldg) * (int64_t)sizeof(HT) < (int64_t(1) << 32);
+        auto kern = a32 ? k_rev_fused<DT, true> : k_rev_fused<DT, false>;
+        static bool attr_set[2] = {false, false};
+        if (!attr_set[a32]) {
+            (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)ds.caps.max_lds_block);
+            (void)hipGetLastError();
+            attr_set[a32] = true;
+        }
+        kern<<<dim3((unsigned)grid), dim3(kFuseWaves * 64), lds, st>>>(
+            (const int32_t *)pl.d_fent + f.ent0, (const int32_t *)pl.d_fco + f.cfirst,
+            (const uint32_t *)pl.d_cmask + f.cfirst, f.nch, pl.lay.K, f.csize, f.plo, f.phi, f.S,
+            (const int64_t *)pl.d_fprow0 + f.prow, (const PT *)P, ldp, (const HT *)Hc, ldh,
+            (const HT *)Gs, (HT *)Gc, ldg, (HT *)part, nb, ntiles);
+        return;
+    }
+    launch_rev_g<DT, W>(ds.caps, tune, pl, L, P, ldp, Gs, Gc, ldg, nb, st, pl.d_fent);
+    const int64_t V = (int64_t)f.S * kFuseWaves;
+    HT *tots = (HT *)ds.ftot.p;
+    auto blocks = [](int64_t n) {
+        return (unsigned)std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (n + kBlock - 1) / kBlock));
+    };
+    k_rev_fterm_bt<DT><<<dim3(blocks((int64_t)nph * V * f.nslot * nb)), dim3(kBlock), 0, st>>>(
+        (const int2 *)pl.d_fterms, (const int64_t *)pl.d_ftoff + f.toff, f.nslot, nph, V,
+        (const HT *)Hc, ldh, (const HT *)Gs, ldg, tots, nb);
+    k_rev_fwg_bt<DT><<<dim3(blocks((int64_t)nph * f.S * f.nslot * nb)), dim3(kBlock), 0, st>>>(
+        (const int64_t *)pl.d_fprow0 + f.prow, f.S, f.nslot, pl.lay.K, f.plo, nph, tots, (HT *)part, nb);
+}
+
 // Forward walk, then the reverse sweep: out[b] = haf, dP[q][b] = d haf/d P.  The forward keeps
 // every level, or (tune.ckpt) only the checkpoint levels of pl.gl_ckpt; a level that was not
 // kept is recomputed with the rest of its segment from the kept level below when the reverse
@@ -1825,7 +1920,8 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
     if (tune.pairs) EGG_TRY(ensure_items(pl));
     const size_t hb = sizeof(HT), pb = sizeof(typename E::P);
     const int64_t chunk = tune.chunk > 0 ? std::min(tune.chunk, B) : B;
-    const bool par = sizeof(W) == 4 && use_par_dp(ds.caps, pl, DT);
+    const bool fused = sizeof(W) == 4 && use_fused(pl, DT);
+    const bool par = !fused && sizeof(W) == 4 && use_par_dp(ds.caps, pl, DT);
     const GradLevels &gl = tune.ckpt ? pl.gl_ckpt : pl.gl_full;
     // forward level rows: padded to whole tile rows for the 16-byte-load kernel
     const bool pad = tune.lds && tune.vec;
@@ -1833,7 +1929,16 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
                               (pad ? pad_rows(chunk) : chunk) * hb));
     const int64_t ldmax = pad ? pad_rows(chunk) : chunk;
     EGG_TRY(ensure(ds.G, (size_t)2 * lay.max_level * ldmax * hb));
-    EGG_TRY(ensure(ds.part, (size_t)std::max<int64_t>(par ? pl.part_rows + pl.nred : pl.nseg, 1) * chunk * hb));
+    EGG_TRY(ensure(ds.part, (size_t)std::max<int64_t>(fused ? pl.fpart_rows + pl.fnred
+                                                           : par ? pl.part_rows + pl.nred : pl.nseg, 1) *
+                                chunk * hb));
+    if (fused && std::min(chunk, B) < 64) {
+        // small passes: the stripe totals of the largest level
+        size_t tb = 0;
+        for (int L = 2; L <= lay.K / 2 + 1; L++)
+            tb = std::max(tb, fuse_tot_bytes(pl.flv[L], DT) * (size_t)std::min(chunk, B));
+        EGG_TRY(ensure(ds.ftot, std::max<size_t>(tb, 1)));
+    }
     HT *H = (HT *)ds.Hg.p;
     HT *G[2] = {(HT *)ds.G.p, (HT *)ds.G.p + (size_t)lay.max_level * ldmax};
     HT *part = (HT *)ds.part.p;
@@ -1889,6 +1994,11 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
                 while (!gl.kept[seg_lo - 1]) seg_lo--;
                 forward_range(seg_lo, seg_hi);
             }
+            if (fused) {
+                launch_rev_fused<DT, W>(ds, pl, tune, L, Pc, ldp, level_rows(L - 1), ldh, G[L & 1],
+                                        G[(L - 1) & 1], ldh, part, nb, st);
+                continue;
+            }
             if (par) {
                 launch_dp_par<DT>(ds.caps, pl, L, level_rows(L - 1), ldh, G[L & 1], ldh, part, nb, st);
             } else {
@@ -1902,11 +2012,22 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
                 }
             }
             if (L > 2)
-                launch_rev_g<DT, W>(ds.caps, tune, pl, L, Pc, ldp, G[L & 1], G[(L - 1) & 1], ldh, nb, st);
+                launch_rev_g<DT, W>(ds.caps, tune, pl, L, Pc, ldp, G[L & 1], G[(L - 1) & 1], ldh, nb, st,
+                                    pl.d_cent);
         }
         const int64_t total = lay.npairs * nb;
         const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (total + 255) / 256));
-        if (par) {
+        if (fused) {
+            HT *runs = part + (size_t)pl.fpart_rows * nb;
+            const int64_t rtotal = pl.fnred * nb;
+            const int64_t rblocks = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (rtotal + 255) / 256));
+            k_rev_dp_reduce_runs<DT><<<dim3((unsigned)rblocks), dim3(256), 0, st>>>(
+                (const RedRowDev *)pl.d_fred, pl.fnred, (const int32_t *)pl.d_frtask,
+                (const ParTaskDev *)pl.d_ftasks, part, runs, nb);
+            k_rev_dp_reduce_pairs<DT><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
+                (const int32_t *)pl.d_qrow, (const int32_t *)pl.d_qcol, (const int32_t *)pl.d_fsrow0,
+                (const int32_t *)pl.d_fsruns, lay.K, runs, dPc, lddp, lay.npairs, nb);
+        } else if (par) {
             HT *runs = part + (size_t)pl.part_rows * nb;
             const int64_t rtotal = pl.nred * nb;
             const int64_t rblocks = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (rtotal + 255) / 256));
@@ -2847,8 +2968,8 @@ int eggman_cache_clear(void) {
         for (GraphEntry &ge : kv.second.graphs) (void)hipGraphExecDestroy(ge.exec);
         kv.second.graphs.clear();
         DevState &d = kv.second;
-        for (DevBuf *b : {&d.H, &d.Hw, &d.Hg, &d.G, &d.part, &d.A, &d.bnd, &d.Hm, &d.din[0], &d.din[1],
-                          &d.dout[0], &d.dout[1]}) {
+        for (DevBuf *b : {&d.H, &d.Hw, &d.Hg, &d.G, &d.part, &d.A, &d.bnd, &d.Hm, &d.ftot,
+                          &d.din[0], &d.din[1], &d.dout[0], &d.dout[1]}) {
             if (b->p) (void)hipFree(b->p);
             b->p = nullptr;
             b->bytes = 0;
@@ -2864,7 +2985,9 @@ int eggman_cache_clear(void) {
         DevPlan &p = *kv.second;
         for (void *ptr : {p.d_trans, p.d_cent, p.d_coff, p.d_pent, p.d_segs, p.d_qoff, p.d_qseg,
                           p.d_prank, p.d_tasks, p.d_roff, p.d_rtask, p.d_qrow, p.d_qcol,
-                          p.d_red, p.d_srow0, p.d_sruns, (void *)p.d_items})
+                          p.d_red, p.d_srow0, p.d_sruns, p.d_fent, p.d_fco, p.d_cmask, p.d_fprow0,
+                          p.d_fterms, p.d_ftoff, p.d_ftasks, p.d_frtask, p.d_fred, p.d_fsrow0, p.d_fsruns,
+                          (void *)p.d_items})
             if (ptr) (void)hipFree(ptr);
     }
     g_plans.clear();

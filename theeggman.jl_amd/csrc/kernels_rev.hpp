@@ -17,6 +17,7 @@
 // keeps two levels.
 #pragma once
 #include "kernels_fwd.hpp"
+#include "plan_host.hpp"
 
 #pragma clang fp contract(off)
 
@@ -456,6 +457,309 @@ __global__ __launch_bounds__(256) void k_rev_dp_reduce(
         typename E::H acc = E::zero();
         for (int64_t k = qoff[q]; k < qoff[q + 1]; k++) acc = E::add_h(acc, part[(int64_t)qseg[k] * nb + b]);
         dP[q * lddp + b] = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Fused child-major reverse step of one level (Int32 layout, elements of at most 8 bytes): G of
+// the children and the level's dP partial rows in one pass that gathers each parent value once
+// per transition, where the adjoint gather and the parent-major dP gather it twice.  The sums
+// follow the order stated at FusePlanHost (plan_host.hpp); the small-batch kernels below follow
+// it too, so every pass size gives the same bits.
+//
+// f(n) with n an integral_constant equal to `n` (1 <= n <= N).
+template <int N, class F>
+__device__ inline void dispatch_n(int n, F &&f) {
+    if constexpr (N >= 1) {
+        if (n == N) f(std::integral_constant<int, N>{});
+        else dispatch_n<N - 1>(n, f);
+    }
+}
+
+// One workgroup of kFuseWaves waves per (stripe s, 64-instance tile); wave w is wave-stripe
+// v = s * kFuseWaves + w and walks its blocks k = v, v + V, ... in every phase p (the waves of all
+// stripes of a tile sweep the child level together, so neighbouring children's shared parents are
+// read while still in L2), so each child is always handled by the same lanes: G[c] is carried from
+// one phase to the next through Gc.  Per phase: the pair rows (p, p+1 .. K) are staged in LDS; a
+// wave's children of the phase form one stream, taken STEP at a time (at most 16 gathers a step);
+// a step's entry words are loaded two steps ahead and its values one step ahead, every step issuing
+// the same loads, so a wave keeps a step of gathers in flight while it updates the previous one;
+// each child's G term P * G[parent] goes into G[c] and its dP term H[c] * G[parent] into slot
+// j - p - 1 of the wave's LDS total row; at the phase's end the waves' rows are summed in wave order
+// into the stripe's partial row.  A child's parents have distinct partners, so its update reads all
+// its slots before writing any.
+//
+// A32: every row offset of Hc, Gs and Gc (row * row bytes + column bytes) fits in 32 bits, so a
+// row access is a uniform byte base plus one 32-bit per-lane offset (the saddr form of the loads)
+// instead of a 64-bit per-lane address per gather.
+template <int DT, bool A32>
+__global__ __launch_bounds__(kFuseWaves * 64) void k_rev_fused(
+    const int32_t *__restrict__ fent, const int32_t *__restrict__ fco,
+    const uint32_t *__restrict__ cmask, int64_t nch, int K, int csize, int plo, int phi, int S,
+    const int64_t *__restrict__ prow0, const typename Elt<DT>::P *__restrict__ P, int64_t ldp,
+    const typename Elt<DT>::H *__restrict__ Hc, int64_t ldh,
+    const typename Elt<DT>::H *__restrict__ Gs, typename Elt<DT>::H *__restrict__ Gc, int64_t ldg,
+    typename Elt<DT>::H *__restrict__ part, int64_t nb, int64_t ntiles) {
+    using E = Elt<DT>;
+    using HT = typename E::H;
+    using PT = typename E::P;
+    constexpr int bits = Packing<int32_t>::bits;
+    constexpr int32_t mask = (1 << bits) - 1;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int nslot = K - plo;
+    PT *prow = reinterpret_cast<PT *>(lds_raw);
+    HT *rows = reinterpret_cast<HT *>(lds_raw + (size_t)nslot * 64 * sizeof(PT));
+    const int lane = (int)(threadIdx.x & 63);
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    // workgroup j runs on XCD j mod 8: the stripes of a tile share one XCD (and its L2) when there
+    // are 8 tiles or more, else each tile's stripes are dealt to the XCDs that hold the tile
+    const int64_t j = blockIdx.x, xcd = j & 7, r = j >> 3;
+    int64_t tile, s;
+    if (ntiles >= 8) {
+        tile = xcd + 8 * (r / S);
+        s = r % S;
+        if (tile >= ntiles) return;
+    } else {
+        tile = xcd % ntiles;
+        const int64_t g = xcd / ntiles, ng = (8 - tile + ntiles - 1) / ntiles;
+        s = r * ng + g;
+        if (s >= S) return;
+    }
+    HT *tr = rows + (size_t)wave * nslot * 64;   // this wave's total row
+    const int64_t b = tile * 64 + lane;
+    const bool live = b < nb;
+    // lanes past the batch shadow the last instance
+    const uint32_t col = (uint32_t)(live ? b : nb - 1);
+    const uint32_t cbytes = col * (uint32_t)sizeof(HT);
+    const uint32_t hrow = (uint32_t)(ldh * (int64_t)sizeof(HT)), grow = (uint32_t)(ldg * (int64_t)sizeof(HT));
+    // element (row, col) of a level: base + row * row bytes + column bytes
+    auto row_at = [&](const HT *base, int64_t row, int64_t ld, uint32_t rbytes) -> const HT * {
+        if constexpr (A32)
+            return reinterpret_cast<const HT *>(reinterpret_cast<const char *>(base) +
+                                                (uint64_t)((uint32_t)row * rbytes + cbytes));
+        else
+            return base + row * ld + col;
+    };
+    const int64_t V = (int64_t)S * kFuseWaves, v = s * kFuseWaves + wave;
+    const int64_t nblk = (nch + kFuseT - 1) / kFuseT;
+    for (int u = 0; u < nslot; u++) tr[u * 64 + lane] = E::zero();
+    int off = 0;   // start of the phase's run inside every child's sorted list
+    for (int p = plo; p <= phi; p++) {
+        const int nrow = K - p, np = nrow - csize;
+        const int qbase = (p - 1) * K - p * (p + 1) / 2 + p;   // 0-based index of pair (p, p + 1)
+        const uint32_t below = (1u << p) - 1u;                  // elements 1 .. p
+        __syncthreads();
+        for (int u = wave; u < nrow; u += kFuseWaves) prow[u * 64 + lane] = (P + (int64_t)(qbase + u) * ldp)[col];
+        __syncthreads();
+        dispatch_n<16>(np, [&](auto npc) {
+            constexpr int NP = decltype(npc)::value;
+            constexpr int STEP = NP <= 4 ? 4 : NP <= 8 ? 2 : 1;    // children per step (<= 16 gathers)
+            struct Buf {
+                HT h[STEP], g0[STEP], g[STEP][NP];
+            };
+            struct Step {
+                int cv;          // lane q: rank of child q (level-relative)
+                int cov;         // lane q: start of child q's sorted list
+                uint32_t on;     // uniform, bit q: child q is real (else it repeats a real child)
+            };
+            // The wave's children of the phase as one stream: its blocks in groups of 8 (lane l of
+            // a group is child l & 7 of block gk + (l >> 3) V), the next group's masks and offsets
+            // loaded one group ahead; a group's children in the phase are the set bits of a ballot,
+            // in (block, rank) order.
+            auto load_group = [&](int64_t g, uint32_t &cmv, int32_t &cov) {
+                const int64_t kl = g + (int64_t)(lane >> 3) * V;
+                const int64_t cl = kl * kFuseT + (lane & 7);
+                const bool in = kl < nblk && cl < nch;
+                cmv = in ? cmask[cl] : 0xffffffffu;
+                cov = in ? fco[cl] : 0;
+            };
+            int64_t gk = v, gkn = v + 8 * V;
+            uint32_t cmc, cmn;
+            int32_t coc, con;
+            load_group(gk, cmc, coc);
+            load_group(gkn, cmn, con);
+            uint64_t rem = __ballot((cmc & below) == 0);
+            int lastc = 0, lastco = 0;
+            auto take = [&](int &c, int &co) -> bool {
+                while (rem == 0) {
+                    if (gkn >= nblk) return false;
+                    gk = gkn;
+                    coc = con;
+                    rem = __ballot((cmn & below) == 0);
+                    gkn = gk + 8 * V;
+                    load_group(gkn, cmn, con);
+                }
+                const int li = __builtin_ctzll(rem);
+                rem &= rem - 1;
+                c = (int)((gk + (int64_t)(li >> 3) * V) * kFuseT + (li & 7));
+                co = __builtin_amdgcn_readlane(coc, li);
+                lastc = c;
+                lastco = co;
+                return true;
+            };
+            // the next STEP children of the stream (past its end, repeats of the last child)
+            auto form = [&](Step &d) {
+                d.on = 0;
+                d.cv = lastc;
+                d.cov = lastco;
+#pragma unroll
+                for (int q = 0; q < STEP; q++) {
+                    int c = lastc, co = lastco;
+                    if (take(c, co)) d.on |= 1u << q;
+                    d.cv = lane == q ? c : d.cv;
+                    d.cov = lane == q ? co : d.cov;
+                }
+            };
+            // a step's entry words: lane q * NP + u holds entry u of child q's phase run
+            auto entries = [&](const Step &d) -> int {
+                const int q = min(lane / NP, STEP - 1);
+                const int co = __shfl(d.cov, q);
+                return fent[co + off + (lane / NP < STEP ? lane % NP : NP - 1)];
+            };
+            // a step's loads, the same set every step: H, carried G and the NP parent values of
+            // each child
+            auto issue = [&](const Step &d, int ew, Buf &bf) {
+#pragma unroll
+                for (int q = 0; q < STEP; q++) {
+                    const int c = __builtin_amdgcn_readlane(d.cv, q);
+                    bf.h[q] = *row_at(Hc, c, ldh, hrow);
+                    bf.g0[q] = p > plo ? *row_at(Gc, c, ldg, grow) : E::zero();
+#pragma unroll
+                    for (int u = 0; u < NP; u++)
+                        bf.g[q][u] = *row_at(Gs, __builtin_amdgcn_readlane(ew, q * NP + u) >> bits, ldg, grow);
+                }
+            };
+            // a step's updates, children in order: G term in list order, dP terms into the
+            // child's distinct slots
+            auto update = [&](const Step &d, int ew, const Buf &bf) {
+#pragma unroll
+                for (int q = 0; q < STEP; q++) {
+                    if ((d.on >> q) & 1u) {
+                        HT acc = bf.g0[q], a[NP];
+                        int sl[NP];
+#pragma unroll
+                        for (int u = 0; u < NP; u++) {
+                            const int q1 = __builtin_amdgcn_readlane(ew, q * NP + u) & mask;
+                            sl[u] = (q1 - 1 - qbase) * 64 + lane;
+                            a[u] = tr[sl[u]];
+                        }
+#pragma unroll
+                        for (int u = 0; u < NP; u++) {
+                            acc = E::muladd(acc, prow[sl[u]], bf.g[q][u]);
+                            tr[sl[u]] = E::muladd_hh(a[u], bf.h[q], bf.g[q][u]);
+                        }
+                        if (live) *const_cast<HT *>(row_at(Gc, __builtin_amdgcn_readlane(d.cv, q), ldg, grow)) = acc;
+                    }
+                }
+            };
+            if (rem == 0 && gkn >= nblk) return;   // no child of this phase in the stripe
+            // three steps in flight: step n+2's entry words load, step n+1's values load, step n
+            // updates.  Loads are issued in that order so each wait leaves the later ones pending.
+            Step d0, d1, d2;
+            form(d0);
+            if (!d0.on) return;
+            const int e0i = entries(d0);
+            form(d1);
+            int e0 = e0i, e1 = entries(d1), e2;
+            Buf b0, b1;
+            issue(d0, e0, b0);
+            auto iter = [&](const Buf &cur, Buf &nxt) {
+                form(d2);
+                e2 = entries(d2);
+                issue(d1, e1, nxt);
+                update(d0, e0, cur);
+                d0 = d1;
+                e0 = e1;
+                d1 = d2;
+                e1 = e2;
+            };
+            for (;;) {
+                iter(b0, b1);
+                if (!d0.on) break;
+                iter(b1, b0);
+                if (!d0.on) break;
+            }
+        });
+        // the stripe's partial row: the waves' rows in wave order
+        __syncthreads();
+        const int64_t rb = prow0[p - plo] + s * nrow;
+        for (int e = (int)threadIdx.x; e < nrow * 64; e += kFuseWaves * 64) {
+            HT sum = rows[e];
+#pragma unroll
+            for (int w = 1; w < kFuseWaves; w++) sum = E::add_h(sum, rows[(size_t)w * nslot * 64 + e]);
+            const int64_t bb = tile * 64 + (e & 63);
+            if (bb < nb) part[(rb + (e >> 6)) * nb + bb] = sum;
+        }
+        __syncthreads();
+        for (int u = 0; u < nrow; u++) tr[u * 64 + lane] = E::zero();
+        off += np;
+    }
+}
+
+// Small batches (under 64 instances), the same order.  Totals: one thread per (phase, wave-stripe
+// v, slot u, instance) sums its list of terms (child, parent), loads issued eight at a time;
+// tots[((ph * V + v) * nslot + u) * nb + b].
+template <int DT>
+__global__ __launch_bounds__(kBlock) void k_rev_fterm_bt(
+    const int2 *__restrict__ terms, const int64_t *__restrict__ toff, int nslot, int nph, int64_t V,
+    const typename Elt<DT>::H *__restrict__ Hc, int64_t ldh,
+    const typename Elt<DT>::H *__restrict__ Gs, int64_t ldg, typename Elt<DT>::H *__restrict__ tots,
+    int64_t nb) {
+    using E = Elt<DT>;
+    using HT = typename E::H;
+    const int64_t total = (int64_t)nph * V * nslot * nb;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const int64_t b = i % nb, rest = i / nb;
+        const int u = (int)(rest % nslot);
+        const int64_t pv = rest / nslot;
+        const int64_t v = pv % V;
+        const int ph = (int)(pv / V);
+        const int64_t li = ((int64_t)ph * nslot + u) * V + v;
+        const int64_t t0 = toff[li], t1 = toff[li + 1];
+        HT tot = E::zero();
+        auto batch = [&](int64_t t, auto n) {
+            constexpr int N = decltype(n)::value;
+            HT h[N], g[N];
+#pragma unroll
+            for (int w = 0; w < N; w++) {
+                const int2 cs = terms[t + w];
+                h[w] = Hc[(int64_t)cs.x * ldh + b];
+                g[w] = Gs[(int64_t)cs.y * ldg + b];
+            }
+#pragma unroll
+            for (int w = 0; w < N; w++) tot = E::muladd_hh(tot, h[w], g[w]);
+        };
+        int64_t t = t0;
+        for (; t + 8 <= t1; t += 8) batch(t, std::integral_constant<int, 8>{});
+        batch_tail<8>((int)(t1 - t), [&](auto n) { batch(t, n); });
+        tots[i] = tot;
+    }
+}
+
+// Stripe sums: one thread per (phase, stripe s, slot u, instance) adds the totals of the stripe's
+// kFuseWaves wave-stripes in order and writes the partial row (rows prow0[ph] + s (K - p) + u).
+template <int DT>
+__global__ __launch_bounds__(kBlock) void k_rev_fwg_bt(
+    const int64_t *__restrict__ prow0, int S, int nslot, int K, int plo, int nph,
+    const typename Elt<DT>::H *__restrict__ tots, typename Elt<DT>::H *__restrict__ part, int64_t nb) {
+    using E = Elt<DT>;
+    const int64_t V = (int64_t)S * kFuseWaves;
+    const int64_t total = (int64_t)nph * S * nslot * nb;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const int64_t b = i % nb, rest = i / nb;
+        const int u = (int)(rest % nslot);
+        const int64_t ps = rest / nslot;
+        const int64_t s = ps % S;
+        const int ph = (int)(ps / S);
+        const int nrow = K - plo - ph;
+        if (u >= nrow) continue;
+        const int64_t t0 = (((int64_t)ph * V + s * kFuseWaves) * nslot + u) * nb + b;
+        typename E::H sum = tots[t0];
+#pragma unroll
+        for (int w = 1; w < kFuseWaves; w++) sum = E::add_h(sum, tots[t0 + (int64_t)w * nslot * nb]);
+        part[(prow0[ph] + s * nrow + u) * nb + b] = sum;
     }
 }
 

@@ -133,6 +133,34 @@ int build_plan_host(const PlanLayout &lay, void *starts, void *trans, void *leve
                                  nthreads);
 }
 
+// Two-stage reduction tables of partial rows.  row_tasks[p] lists the tasks of dP row p in their
+// summation order; stage one sums runs of at most `fan_in` consecutive tasks, one stage-one row
+// per (run, column u); stage two sums a pair's stage-one rows in run order.
+static void build_reduction(int K, const std::vector<std::vector<int32_t>> &row_tasks, int fan_in,
+                            std::vector<int64_t> *roff, std::vector<int32_t> *rtask,
+                            std::vector<RedRow> *red, std::vector<int32_t> *srow0,
+                            std::vector<int32_t> *sruns) {
+    roff->assign(K + 2, 0);
+    rtask->clear();
+    for (int p = 1; p <= K; p++) {
+        (*roff)[p] = (int64_t)rtask->size();
+        rtask->insert(rtask->end(), row_tasks[p].begin(), row_tasks[p].end());
+    }
+    (*roff)[K + 1] = (int64_t)rtask->size();
+    red->clear();
+    srow0->assign(K + 2, 0);
+    sruns->assign(K + 2, 0);
+    for (int p = 1; p <= K; p++) {
+        (*srow0)[p] = (int32_t)red->size();
+        const int64_t tb = (*roff)[p], te = (*roff)[p + 1];
+        for (int64_t a = tb; a < te; a += fan_in) {
+            (*sruns)[p]++;
+            for (int u = 0; u < K - p; u++)
+                red->push_back(RedRow{(int32_t)a, (int32_t)std::min<int64_t>(te, a + fan_in), u, 0});
+        }
+    }
+}
+
 template <typename W>
 static int build_rev_t(const PlanLayout &lay, const W *trans, int seg_len, int task_len,
                        int fan_in, RevPlanHost *rp) {
@@ -248,26 +276,7 @@ static int build_rev_t(const PlanLayout &lay, const W *trans, int seg_len, int t
     }
     rp->lvl_task[lay.nlevels - 1] = (int64_t)rp->tasks.size();
     rp->part_rows = rows;
-    rp->roff.assign(K + 2, 0);
-    rp->rtask.clear();
-    for (int p = 1; p <= K; p++) {
-        rp->roff[p] = (int64_t)rp->rtask.size();
-        rp->rtask.insert(rp->rtask.end(), row_tasks[p].begin(), row_tasks[p].end());
-    }
-    rp->roff[K + 1] = (int64_t)rp->rtask.size();
-    // two-stage reduction of the partial rows
-    rp->red.clear();
-    rp->srow0.assign(K + 2, 0);
-    rp->sruns.assign(K + 2, 0);
-    for (int p = 1; p <= K; p++) {
-        rp->srow0[p] = (int32_t)rp->red.size();
-        const int64_t tb = rp->roff[p], te = rp->roff[p + 1];
-        for (int64_t a = tb; a < te; a += fan_in) {
-            rp->sruns[p]++;
-            for (int u = 0; u < K - p; u++)
-                rp->red.push_back(RedRow{(int32_t)a, (int32_t)std::min<int64_t>(te, a + fan_in), u, 0});
-        }
-    }
+    build_reduction(K, row_tasks, fan_in, &rp->roff, &rp->rtask, &rp->red, &rp->srow0, &rp->sruns);
     return EGGMAN_OK;
 }
 
@@ -418,6 +427,136 @@ int validate_plan(const PlanLayout &lay, const void *starts, const void *trans, 
                                    (const int32_t *)levels, nthreads);
     return validate_t<int64_t>(lay, (const int64_t *)starts, (const int64_t *)trans,
                                (const int64_t *)levels, nthreads);
+}
+
+int fuse_stripes(int K, int plo, int64_t nblk) {
+    const int nslot = K - plo;
+    // LDS of a workgroup in FP64: the phase's pair rows and one dP row set per wave, 64 lanes
+    const int64_t lds = (int64_t)(1 + kFuseWaves) * nslot * 64 * 8;
+    const int per_cu = (int)std::max<int64_t>(2, std::min<int64_t>(3, 163840 / std::max<int64_t>(lds, 1)));
+    const int64_t want = (nblk + 4 * kFuseWaves - 1) / (4 * kFuseWaves);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(32 * per_cu, want));
+}
+
+int build_fuse_plan(const PlanLayout &lay, const RevPlanHost &rp, int fan_in, FusePlanHost *fp) {
+    if (lay.index_bytes != 4) return fail(EGGMAN_EINVAL, "the fused reverse step needs the Int32 layout");
+    if (fan_in < 1) return fail(EGGMAN_EINVAL, "fan-in must be positive");
+    const uint64_t(*C)[kMaxK + 1] = kHostBinom.c;
+    const int K = lay.K;
+    const int64_t mask = ((int64_t)1 << lay.bits) - 1;
+    const int32_t *cent = (const int32_t *)rp.cent.data();
+    // each child's parent entries sorted by pair; a child has at most one parent per pair
+    fp->fent.assign(cent, cent + lay.ntrans);
+    fp->fco.assign(lay.nstates, 0);
+    for (int L = 1; L <= K / 2; L++) {
+        const int64_t c0 = lay.levels[L - 1] - 1, c1 = c0 + lay.nst[L - 1];
+        for (int64_t c = c0; c < c1; c++) {
+            std::sort(fp->fent.begin() + rp.coff[c], fp->fent.begin() + rp.coff[c + 1],
+                      [&](int32_t a, int32_t b) { return (a & mask) < (b & mask); });
+            fp->fco[c] = (int32_t)(rp.coff[c] - rp.coff[c0]);
+        }
+    }
+    // element masks
+    fp->cmask.assign(lay.nstates, 0);
+    for (int L = 2; L <= K / 2 + 1; L++) {
+        const int m = lay.size[L - 1], lo = lay.lo[L - 1];
+        const int64_t s0 = lay.levels[L - 1] - 1;
+        int x[kMaxK];
+        colex_unrank(C, K - lo + 1, m, 0, x);
+        for (int64_t r = 0; r < lay.nst[L - 1]; r++) {
+            uint32_t bm = 0;
+            for (int i = 0; i < m; i++) bm |= 1u << (lo + x[i] - 1);
+            fp->cmask[s0 + r] = bm;
+            if (r + 1 < lay.nst[L - 1]) colex_next(m, x);
+        }
+    }
+    // levels, phases and partial rows
+    fp->lv.assign(K / 2 + 2, FuseLevel{});
+    fp->prow0.clear();
+    fp->tasks.clear();
+    std::vector<std::vector<int32_t>> row_tasks(K + 1);
+    int64_t rows = 0;
+    for (int L = 2; L <= K / 2 + 1; L++) {
+        FuseLevel &f = fp->lv[L];
+        f.plo = lay.lo[L - 1];
+        f.phi = 2 * f.plo - 1;
+        f.nslot = K - f.plo;
+        f.csize = lay.size[L - 2];
+        f.nch = lay.nst[L - 2];
+        f.nblk = (f.nch + kFuseT - 1) / kFuseT;
+        f.cfirst = lay.levels[L - 2] - 1;
+        f.ent0 = rp.coff[f.cfirst];
+        f.S = fuse_stripes(K, f.plo, f.nblk);
+        f.prow = (int64_t)fp->prow0.size();
+        for (int p = f.plo; p <= f.phi; p++) {
+            const int nrow = K - p;
+            if (nrow - f.csize < 1) return fail(EGGMAN_EINVAL, "phase %d of level %d has no parents", p, L);
+            fp->prow0.push_back(rows);
+            for (int s = 0; s < f.S; s++) {
+                ParTask tk;
+                tk.start = 0;
+                tk.row0 = rows + (int64_t)s * nrow;
+                tk.len = 0;
+                tk.p = p;
+                tk.level = L;
+                tk.pad = 0;
+                row_tasks[p].push_back((int32_t)fp->tasks.size());
+                fp->tasks.push_back(tk);
+            }
+            rows += (int64_t)f.S * nrow;
+        }
+    }
+    fp->part_rows = rows;
+    build_reduction(K, row_tasks, fan_in, &fp->roff, &fp->rtask, &fp->red, &fp->srow0, &fp->sruns);
+
+    // the terms of every (level, phase, slot, wave-stripe) in summation order: count, then fill,
+    // both visiting children in rank order
+    fp->toff.clear();
+    fp->terms.assign((size_t)2 * lay.ntrans, 0);
+    int64_t pos = 0;
+    for (int L = 2; L <= K / 2 + 1; L++) {
+        FuseLevel &f = fp->lv[L];
+        const int nph = f.phi - f.plo + 1;
+        const int64_t V = (int64_t)f.S * kFuseWaves;
+        const int64_t nlist = (int64_t)nph * f.nslot * V;
+        f.toff = (int64_t)fp->toff.size();
+        std::vector<int64_t> cnt(nlist + 1, 0);
+        auto visit = [&](auto &&emit) {
+            for (int64_t c = 0; c < f.nch; c++) {
+                const uint32_t cm = fp->cmask[f.cfirst + c];
+                const int64_t v = (c / kFuseT) % V;
+                const int32_t *lst = fp->fent.data() + f.ent0 + fp->fco[f.cfirst + c];
+                int off = 0;
+                for (int p = f.plo; p <= f.phi; p++) {
+                    const int np = K - p - f.csize;
+                    if ((cm & ((1u << p) - 1u)) != 0) break;     // min(c) <= p: no later phase either
+                    for (int u = 0; u < np; u++) {
+                        const int32_t e = lst[off + u];
+                        const int q = (int)(e & mask);
+                        // slot of pair (p, j): j - p - 1 = q - 1 - (0-based index of (p, p + 1))
+                        const int slot = q - 1 - (int)(pair_index(p, p + 1, K) - 1);
+                        emit(((int64_t)(p - f.plo) * f.nslot + slot) * V + v, (int32_t)c,
+                             (int32_t)(e >> lay.bits));
+                    }
+                    off += np;
+                }
+            }
+        };
+        visit([&](int64_t li, int32_t, int32_t) { cnt[li + 1]++; });
+        for (int64_t i = 0; i < nlist; i++) cnt[i + 1] += cnt[i];
+        std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+        visit([&](int64_t li, int32_t c, int32_t sr) {
+            const int64_t at = pos + fill[li]++;
+            fp->terms[(size_t)2 * at] = c;
+            fp->terms[(size_t)2 * at + 1] = sr;
+        });
+        for (int64_t i = 0; i < nlist; i++) fp->toff.push_back(pos + cnt[i]);
+        pos += cnt[nlist];
+    }
+    fp->toff.push_back(pos);
+    if (pos != lay.ntrans) return fail(EGGMAN_EINVAL, "fused term lists hold %lld of %lld transitions",
+                                       (long long)pos, (long long)lay.ntrans);
+    return EGGMAN_OK;
 }
 
 }  // namespace eggman

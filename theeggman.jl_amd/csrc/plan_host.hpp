@@ -82,6 +82,66 @@ struct RevPlanHost {
 int build_rev_plan(const PlanLayout &lay, const void *trans, int seg_len, int task_len,
                    int fan_in, RevPlanHost *rp);
 
+// Fused child-major reverse step (kernels_rev.hpp: k_rev_fused), Int32 layout.  Level L holds
+// all m-subsets of {lo..K} (plan_rank.hpp), so the parents of a child c of level L-1 are exactly
+// c + {p, j} with lo <= p < min(c) and j > p, j not in c: for each p ("phase") there are
+// n_p = K - p - (m - 2) of them, and every child with min(c) > p has all n_p.  A child's parent
+// list sorted by pair, (p, j) ascending, therefore holds its phases one after another, phase p
+// starting at the same offset sum_{p' < p} n_p' in every list.
+//
+// The order every fused kernel sums dP in, a function of the plan only:
+//   * children are cut into blocks of kFuseT consecutive ranks; block k belongs to wave-stripe
+//     k mod V, V = S * kFuseWaves, and wave-stripe v to workgroup stripe v / kFuseWaves;
+//   * total(p, v)[u] = the terms H[c] * G[parent] of slot u = j - p - 1 over the children c of
+//     stripe v's blocks with min(c) > p, blocks ascending and ranks ascending within a block,
+//     added one after another from zero;
+//   * the stripe's partial row (p, s)[u] = total(p, 4s) + total(p, 4s+1) + ... in wave order;
+//   * dP[(p, j)] = the partial rows of row p, levels ascending, stripes ascending, summed in
+//     runs of `fan_in` and then over the runs (as the parent-major tasks are).
+// And G[c] = the terms P[(p, j)] * G[parent] of c's sorted list, in list order, from zero.
+// For small batches the terms of each (phase, slot, wave-stripe) are also listed in that order
+// (`terms`), so one thread can sum them without visiting the other children.
+constexpr int kFuseT = 8;        // consecutive child ranks per block
+constexpr int kFuseWaves = 4;    // wave-stripes per workgroup stripe
+
+struct FuseLevel {
+    int S = 0;                   // workgroup stripes
+    int plo = 0, phi = 0;        // phases p = plo .. phi
+    int nslot = 0;               // K - plo: slots of the widest phase
+    int csize = 0;               // subset size of the children
+    int64_t nch = 0, nblk = 0;   // children, blocks
+    int64_t cfirst = 0;          // 0-based first child state
+    int64_t ent0 = 0;            // first entry of the children's sorted lists
+    int64_t prow = 0;            // index in prow0 of phase plo
+    int64_t toff = 0;            // index in toff of (phase plo, slot 0, wave-stripe 0)
+};
+
+struct FusePlanHost {
+    std::vector<int32_t> fent;    // ntrans: each child's parent entries sorted by pair
+    std::vector<int32_t> fco;     // nstates: start of a child's list, relative to its level's first
+    std::vector<uint32_t> cmask;  // nstates: bit e-1 set for each element e
+    std::vector<FuseLevel> lv;    // index L (the parents' level), 2 .. K/2 + 1
+    std::vector<int64_t> prow0;   // first partial row of each (level, phase)
+    // the terms of each (level, phase, slot u < nslot, wave-stripe v), in summation order, as
+    // (child rank, parent rank) within their levels: terms[toff[i] .. toff[i + 1]), i = lv.toff +
+    // (phase * nslot + u) * V + v
+    std::vector<int32_t> terms;   // pairs of int32
+    std::vector<int64_t> toff;
+    int64_t part_rows = 0;
+    std::vector<ParTask> tasks;   // one per (level, phase, stripe): row0, p, level
+    std::vector<int32_t> rtask;   // tasks of each row p, levels then stripes ascending
+    std::vector<int64_t> roff;    // index p, 1..K
+    std::vector<RedRow> red;
+    std::vector<int32_t> srow0, sruns;
+};
+
+// Workgroup stripes of a level: about 16 blocks per wave-stripe, at most one XCD's worth of
+// resident workgroups (32 CUs x 3, or x 2 when the FP64 rows of the widest phase need more than a
+// third of a CU's 160 KB of LDS).  Fixed per degree, so the dP order is too.
+int fuse_stripes(int K, int plo, int64_t nblk);
+
+int build_fuse_plan(const PlanLayout &lay, const RevPlanHost &rp, int fan_in, FusePlanHost *fp);
+
 // Where the gradient keeps the forward values of levels 1 .. K/2 (level L's values are read by
 // the reverse step at L + 1; the top level's value is the result).  A kept level stays resident
 // in the kept region at row `off[L]`; the other levels form segments (maximal runs of levels
