@@ -142,7 +142,6 @@ struct DevState {
     DevBuf A;                        // staged matrix (repeated-pattern batches)
     DevBuf bnd;                      // per-instance range bounds of exact device-pointer calls
     DevBuf Hm;                       // the walks of a mixed call's large groups, one slice each
-    DevBuf ftot;                     // small-batch stripe totals of the fused reverse step
     // side streams of mixed calls (default priority), forked from and joined to the call's stream
     static constexpr int kSide = 4;
     hipStream_t side[kSide] = {};
@@ -156,7 +155,7 @@ struct DevState {
     DevBuf din[2], dout[2];
     size_t bytes() const {
         return H.bytes + Hw.bytes + Hg.bytes + G.bytes + part.bytes + A.bytes + bnd.bytes + Hm.bytes +
-               ftot.bytes + din[0].bytes + din[1].bytes + dout[0].bytes + dout[1].bytes;
+               din[0].bytes + din[1].bytes + dout[0].bytes + dout[1].bytes;
     }
 };
 
@@ -1852,14 +1851,11 @@ static void launch_rev_g(const DevCaps &caps, const Tune &tune, const DevPlan &p
         (typename E::H *)Gc, nb, map);
 }
 
-// Stripe totals of the small-batch fused step: one per (phase, wave-stripe, slot, instance).
-static size_t fuse_tot_bytes(const FuseLevel &f, int dtype) {
-    return (size_t)(f.phi - f.plo + 1) * f.S * kFuseWaves * f.nslot * elt_hbytes(dtype);
-}
-
 // The reverse step of level L by the fused order: G of level L-1 into Gc and the level's dP
-// partial rows.  64 instances or more take k_rev_fused; fewer take the adjoint gather over the
-// sorted lists and the term-list and stripe-sum kernels (totals in ds.ftot, sized by the caller).
+// partial rows.  Batches of 64 instances or more take k_rev_fused, except on levels whose
+// wave-stripes hold at most four blocks: there, and for smaller batches, the phase loop would be
+// bound by latency, so the adjoint gather over the sorted lists and k_rev_fstripe run every phase
+// at once (the same sums in the same order).
 template <int DT, typename W>
 static void launch_rev_fused(DevState &ds, const DevPlan &pl, const Tune &tune, int L, const void *P,
                              int64_t ldp, const void *Hc, int64_t ldh, const void *Gs, void *Gc,
@@ -1869,7 +1865,8 @@ static void launch_rev_fused(DevState &ds, const DevPlan &pl, const Tune &tune, 
     using PT = typename E::P;
     const FuseLevel &f = pl.flv[L];
     const int nph = f.phi - f.plo + 1;
-    if (nb >= 64) {
+    const int64_t V = (int64_t)f.S * kFuseWaves;
+    if (nb >= 64 && f.nblk > 4 * V) {
         const int64_t ntiles = (nb + 63) / 64;
         const size_t lds = (size_t)f.nslot * 64 * (sizeof(PT) + kFuseWaves * sizeof(HT));
         const int64_t grid = ntiles >= 8 ? 8 * (int64_t)f.S * ((ntiles + 7) / 8)
@@ -1893,16 +1890,12 @@ static void launch_rev_fused(DevState &ds, const DevPlan &pl, const Tune &tune, 
         return;
     }
     launch_rev_g<DT, W>(ds.caps, tune, pl, L, P, ldp, Gs, Gc, ldg, nb, st, pl.d_fent);
-    const int64_t V = (int64_t)f.S * kFuseWaves;
-    HT *tots = (HT *)ds.ftot.p;
-    auto blocks = [](int64_t n) {
-        return (unsigned)std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (n + kBlock - 1) / kBlock));
-    };
-    k_rev_fterm_bt<DT><<<dim3(blocks((int64_t)nph * V * f.nslot * nb)), dim3(kBlock), 0, st>>>(
-        (const int2 *)pl.d_fterms, (const int64_t *)pl.d_ftoff + f.toff, f.nslot, nph, V,
-        (const HT *)Hc, ldh, (const HT *)Gs, ldg, tots, nb);
-    k_rev_fwg_bt<DT><<<dim3(blocks((int64_t)nph * f.S * f.nslot * nb)), dim3(kBlock), 0, st>>>(
-        (const int64_t *)pl.d_fprow0 + f.prow, f.S, f.nslot, pl.lay.K, f.plo, nph, tots, (HT *)part, nb);
+    const int64_t n = (int64_t)nph * f.S * f.nslot * nb * kFuseWaves;
+    k_rev_fstripe<DT><<<dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (n + kBlock - 1) / kBlock))),
+                        dim3(kBlock), 0, st>>>(
+        (const int2 *)pl.d_fterms, (const int64_t *)pl.d_ftoff + f.toff,
+        (const int64_t *)pl.d_fprow0 + f.prow, f.S, f.nslot, pl.lay.K, f.plo, nph, (const HT *)Hc, ldh,
+        (const HT *)Gs, ldg, (HT *)part, nb);
 }
 
 // Forward walk, then the reverse sweep: out[b] = haf, dP[q][b] = d haf/d P.  The forward keeps
@@ -1932,13 +1925,6 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
     EGG_TRY(ensure(ds.part, (size_t)std::max<int64_t>(fused ? pl.fpart_rows + pl.fnred
                                                            : par ? pl.part_rows + pl.nred : pl.nseg, 1) *
                                 chunk * hb));
-    if (fused && std::min(chunk, B) < 64) {
-        // small passes: the stripe totals of the largest level
-        size_t tb = 0;
-        for (int L = 2; L <= lay.K / 2 + 1; L++)
-            tb = std::max(tb, fuse_tot_bytes(pl.flv[L], DT) * (size_t)std::min(chunk, B));
-        EGG_TRY(ensure(ds.ftot, std::max<size_t>(tb, 1)));
-    }
     HT *H = (HT *)ds.Hg.p;
     HT *G[2] = {(HT *)ds.G.p, (HT *)ds.G.p + (size_t)lay.max_level * ldmax};
     HT *part = (HT *)ds.part.p;
@@ -2968,7 +2954,7 @@ int eggman_cache_clear(void) {
         for (GraphEntry &ge : kv.second.graphs) (void)hipGraphExecDestroy(ge.exec);
         kv.second.graphs.clear();
         DevState &d = kv.second;
-        for (DevBuf *b : {&d.H, &d.Hw, &d.Hg, &d.G, &d.part, &d.A, &d.bnd, &d.Hm, &d.ftot,
+        for (DevBuf *b : {&d.H, &d.Hw, &d.Hg, &d.G, &d.part, &d.A, &d.bnd, &d.Hm,
                           &d.din[0], &d.din[1], &d.dout[0], &d.dout[1]}) {
             if (b->p) (void)hipFree(b->p);
             b->p = nullptr;

@@ -467,6 +467,22 @@ __global__ __launch_bounds__(256) void k_rev_dp_reduce(
 // follow the order stated at FusePlanHost (plan_host.hpp); the small-batch kernels below follow
 // it too, so every pass size gives the same bits.
 //
+// A state value of another lane of the wave.
+template <int DT>
+__device__ inline typename Elt<DT>::H shfl_h(typename Elt<DT>::H x, int src) {
+    using HT = typename Elt<DT>::H;
+    if constexpr (std::is_same<HT, float>::value || std::is_same<HT, double>::value) {
+        return __shfl(x, src);
+    } else if constexpr (std::is_same<HT, float2>::value) {
+        return make_float2(__shfl(x.x, src), __shfl(x.y, src));
+    } else if constexpr (std::is_same<HT, double2>::value) {
+        return make_double2(__shfl(x.x, src), __shfl(x.y, src));
+    } else {
+        return make_ulonglong2((unsigned long long)__shfl((long long)x.x, src),
+                               (unsigned long long)__shfl((long long)x.y, src));
+    }
+}
+
 // f(n) with n an integral_constant equal to `n` (1 <= n <= N).
 template <int N, class F>
 __device__ inline void dispatch_n(int n, F &&f) {
@@ -654,13 +670,14 @@ __global__ __launch_bounds__(kFuseWaves * 64) void k_rev_fused(
             };
             if (rem == 0 && gkn >= nblk) return;   // no child of this phase in the stripe
             // three steps in flight: step n+2's entry words load, step n+1's values load, step n
-            // updates.  Loads are issued in that order so each wait leaves the later ones pending.
+            // updates.  Each step issues its loads in that order, so every wait leaves the later
+            // ones pending.
             Step d0, d1, d2;
             form(d0);
             if (!d0.on) return;
-            const int e0i = entries(d0);
+            int e0 = entries(d0);
             form(d1);
-            int e0 = e0i, e1 = entries(d1), e2;
+            int e1 = entries(d1), e2;
             Buf b0, b1;
             issue(d0, e0, b0);
             auto iter = [&](const Buf &cur, Buf &nxt) {
@@ -696,70 +713,63 @@ __global__ __launch_bounds__(kFuseWaves * 64) void k_rev_fused(
     }
 }
 
-// Small batches (under 64 instances), the same order.  Totals: one thread per (phase, wave-stripe
-// v, slot u, instance) sums its list of terms (child, parent), loads issued eight at a time;
-// tots[((ph * V + v) * nslot + u) * nb + b].
+// The same order without the phase loop, for small batches and small levels.  Four adjacent
+// threads take the kFuseWaves wave-stripes of one (phase, stripe s, slot u, instance): each sums
+// its term list (child, parent), loads issued eight at a time, and the first adds the four totals
+// in wave order (shuffles within the wave) and writes the stripe's partial row
+// (rows prow0[ph] + s (K - p) + u).  G comes from the adjoint gather over the sorted lists
+// (k_rev_g / k_rev_g_bt), which sums each child's list in order.
 template <int DT>
-__global__ __launch_bounds__(kBlock) void k_rev_fterm_bt(
-    const int2 *__restrict__ terms, const int64_t *__restrict__ toff, int nslot, int nph, int64_t V,
-    const typename Elt<DT>::H *__restrict__ Hc, int64_t ldh,
-    const typename Elt<DT>::H *__restrict__ Gs, int64_t ldg, typename Elt<DT>::H *__restrict__ tots,
-    int64_t nb) {
+__global__ __launch_bounds__(kBlock) void k_rev_fstripe(
+    const int2 *__restrict__ terms, const int64_t *__restrict__ toff, const int64_t *__restrict__ prow0,
+    int S, int nslot, int K, int plo, int nph, const typename Elt<DT>::H *__restrict__ Hc,
+    int64_t ldh, const typename Elt<DT>::H *__restrict__ Gs, int64_t ldg,
+    typename Elt<DT>::H *__restrict__ part, int64_t nb) {
     using E = Elt<DT>;
     using HT = typename E::H;
-    const int64_t total = (int64_t)nph * V * nslot * nb;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-        const int64_t b = i % nb, rest = i / nb;
-        const int u = (int)(rest % nslot);
-        const int64_t pv = rest / nslot;
-        const int64_t v = pv % V;
-        const int ph = (int)(pv / V);
-        const int64_t li = ((int64_t)ph * nslot + u) * V + v;
-        const int64_t t0 = toff[li], t1 = toff[li + 1];
-        HT tot = E::zero();
-        auto batch = [&](int64_t t, auto n) {
-            constexpr int N = decltype(n)::value;
-            HT h[N], g[N];
-#pragma unroll
-            for (int w = 0; w < N; w++) {
-                const int2 cs = terms[t + w];
-                h[w] = Hc[(int64_t)cs.x * ldh + b];
-                g[w] = Gs[(int64_t)cs.y * ldg + b];
-            }
-#pragma unroll
-            for (int w = 0; w < N; w++) tot = E::muladd_hh(tot, h[w], g[w]);
-        };
-        int64_t t = t0;
-        for (; t + 8 <= t1; t += 8) batch(t, std::integral_constant<int, 8>{});
-        batch_tail<8>((int)(t1 - t), [&](auto n) { batch(t, n); });
-        tots[i] = tot;
-    }
-}
-
-// Stripe sums: one thread per (phase, stripe s, slot u, instance) adds the totals of the stripe's
-// kFuseWaves wave-stripes in order and writes the partial row (rows prow0[ph] + s (K - p) + u).
-template <int DT>
-__global__ __launch_bounds__(kBlock) void k_rev_fwg_bt(
-    const int64_t *__restrict__ prow0, int S, int nslot, int K, int plo, int nph,
-    const typename Elt<DT>::H *__restrict__ tots, typename Elt<DT>::H *__restrict__ part, int64_t nb) {
-    using E = Elt<DT>;
+    static_assert(kFuseWaves == 4 && kBlock % 4 == 0, "a stripe's wave-stripes are four adjacent lanes");
     const int64_t V = (int64_t)S * kFuseWaves;
-    const int64_t total = (int64_t)nph * S * nslot * nb;
+    const int64_t total = (int64_t)nph * S * nslot * nb * kFuseWaves;   // a multiple of 4
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-        const int64_t b = i % nb, rest = i / nb;
+    const int64_t start = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // every thread of a wave runs the same number of iterations (the shuffles need all four)
+    for (int64_t i0 = start - (threadIdx.x & 63); i0 < total; i0 += stride) {
+        const int64_t i = i0 + (threadIdx.x & 63);
+        const bool in = i < total;
+        const int w = (int)(i & 3);
+        const int64_t q = in ? i >> 2 : 0;
+        const int64_t b = q % nb, rest = q / nb;
         const int u = (int)(rest % nslot);
         const int64_t ps = rest / nslot;
         const int64_t s = ps % S;
         const int ph = (int)(ps / S);
         const int nrow = K - plo - ph;
-        if (u >= nrow) continue;
-        const int64_t t0 = (((int64_t)ph * V + s * kFuseWaves) * nslot + u) * nb + b;
-        typename E::H sum = tots[t0];
+        HT tot = E::zero();
+        if (in && u < nrow) {
+            const int64_t li = ((int64_t)ph * nslot + u) * V + s * kFuseWaves + w;
+            const int64_t t0 = toff[li], t1 = toff[li + 1];
+            auto batch = [&](int64_t t, auto n) {
+                constexpr int N = decltype(n)::value;
+                HT h[N], g[N];
 #pragma unroll
-        for (int w = 1; w < kFuseWaves; w++) sum = E::add_h(sum, tots[t0 + (int64_t)w * nslot * nb]);
-        part[(prow0[ph] + s * nrow + u) * nb + b] = sum;
+                for (int x = 0; x < N; x++) {
+                    const int2 cs = terms[t + x];
+                    h[x] = Hc[(int64_t)cs.x * ldh + b];
+                    g[x] = Gs[(int64_t)cs.y * ldg + b];
+                }
+#pragma unroll
+                for (int x = 0; x < N; x++) tot = E::muladd_hh(tot, h[x], g[x]);
+            };
+            int64_t t = t0;
+            for (; t + 8 <= t1; t += 8) batch(t, std::integral_constant<int, 8>{});
+            batch_tail<8>((int)(t1 - t), [&](auto n) { batch(t, n); });
+        }
+        // the four totals in wave order
+        const int lane = (int)(threadIdx.x & 63), l0 = lane & ~3;
+        HT sum = shfl_h<DT>(tot, l0);
+#pragma unroll
+        for (int x = 1; x < kFuseWaves; x++) sum = E::add_h(sum, shfl_h<DT>(tot, l0 + x));
+        if (in && w == 0 && u < nrow) part[(prow0[ph] + s * nrow + u) * nb + b] = sum;
     }
 }
 
