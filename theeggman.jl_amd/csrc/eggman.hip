@@ -1890,6 +1890,16 @@ static void launch_rev_fused(DevState &ds, const DevPlan &pl, const Tune &tune, 
         return;
     }
     launch_rev_g<DT, W>(ds.caps, tune, pl, L, P, ldp, Gs, Gc, ldg, nb, st, pl.d_fent);
+    if (nb >= 64) {
+        const int64_t ntiles = (nb + 63) / 64;
+        const int64_t nw = (int64_t)nph * f.S * f.nslot * ntiles;
+        k_rev_fstripe_w<DT><<<dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, nw))),
+                              dim3(kFuseWaves * 64), 0, st>>>(
+            (const int2 *)pl.d_fterms, (const int64_t *)pl.d_ftoff + f.toff,
+            (const int64_t *)pl.d_fprow0 + f.prow, f.S, f.nslot, pl.lay.K, f.plo, nph, (const HT *)Hc,
+            ldh, (const HT *)Gs, ldg, (HT *)part, nb, ntiles);
+        return;
+    }
     const int64_t n = (int64_t)nph * f.S * f.nslot * nb * kFuseWaves;
     k_rev_fstripe<DT><<<dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (n + kBlock - 1) / kBlock))),
                         dim3(kBlock), 0, st>>>(

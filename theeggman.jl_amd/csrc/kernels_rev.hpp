@@ -773,4 +773,60 @@ __global__ __launch_bounds__(kBlock) void k_rev_fstripe(
     }
 }
 
+// k_rev_fstripe for batches of 64 or more: one workgroup per (phase, stripe s, slot u,
+// 64-instance tile), wave w summing wave-stripe 4s + w's term list with lanes over instances
+// (uniform terms, whole-row loads), and the four totals added in wave order through LDS.
+template <int DT>
+__global__ __launch_bounds__(kFuseWaves * 64) void k_rev_fstripe_w(
+    const int2 *__restrict__ terms, const int64_t *__restrict__ toff, const int64_t *__restrict__ prow0,
+    int S, int nslot, int K, int plo, int nph, const typename Elt<DT>::H *__restrict__ Hc,
+    int64_t ldh, const typename Elt<DT>::H *__restrict__ Gs, int64_t ldg,
+    typename Elt<DT>::H *__restrict__ part, int64_t nb, int64_t ntiles) {
+    using E = Elt<DT>;
+    using HT = typename E::H;
+    __shared__ HT tw[kFuseWaves][64];
+    const int lane = (int)(threadIdx.x & 63);
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int64_t V = (int64_t)S * kFuseWaves;
+    const int64_t nwork = (int64_t)nph * S * nslot * ntiles;
+    for (int64_t g = blockIdx.x; g < nwork; g += gridDim.x) {
+        const int64_t tile = g % ntiles, rest = g / ntiles;
+        const int u = (int)(rest % nslot);
+        const int64_t ps = rest / nslot;
+        const int64_t s = ps % S;
+        const int ph = (int)(ps / S);
+        const int nrow = K - plo - ph;
+        if (u >= nrow) continue;   // uniform over the workgroup
+        const int64_t b = tile * 64 + lane;
+        const int64_t bl = b < nb ? b : nb - 1;
+        const int64_t li = ((int64_t)ph * nslot + u) * V + s * kFuseWaves + w;
+        const int64_t t0 = toff[li], t1 = toff[li + 1];
+        HT tot = E::zero();
+        auto batch = [&](int64_t t, auto n) {
+            constexpr int N = decltype(n)::value;
+            HT h[N], gv[N];
+#pragma unroll
+            for (int x = 0; x < N; x++) {
+                const int2 cs = terms[t + x];
+                h[x] = Hc[(int64_t)cs.x * ldh + bl];
+                gv[x] = Gs[(int64_t)cs.y * ldg + bl];
+            }
+#pragma unroll
+            for (int x = 0; x < N; x++) tot = E::muladd_hh(tot, h[x], gv[x]);
+        };
+        int64_t t = t0;
+        for (; t + 8 <= t1; t += 8) batch(t, std::integral_constant<int, 8>{});
+        batch_tail<8>((int)(t1 - t), [&](auto n) { batch(t, n); });
+        tw[w][lane] = tot;
+        __syncthreads();
+        if (w == 0) {
+            HT sum = tw[0][lane];
+#pragma unroll
+            for (int x = 1; x < kFuseWaves; x++) sum = E::add_h(sum, tw[x][lane]);
+            if (b < nb) part[(prow0[ph] + s * nrow + u) * nb + b] = sum;
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace eggman

@@ -430,12 +430,12 @@ int validate_plan(const PlanLayout &lay, const void *starts, const void *trans, 
 }
 
 int fuse_stripes(int K, int plo, int64_t nblk) {
-    const int nslot = K - plo;
-    // LDS of a workgroup in FP64: the phase's pair rows and one dP row set per wave, 64 lanes
-    const int64_t lds = (int64_t)(1 + kFuseWaves) * nslot * 64 * 8;
-    const int per_cu = (int)std::max<int64_t>(2, std::min<int64_t>(3, 163840 / std::max<int64_t>(lds, 1)));
+    (void)K;
+    (void)plo;
+    // the fused kernel runs two workgroups per CU (its registers allow two waves per SIMD), so an
+    // XCD (32 CUs) holds 64 stripes of a tile at once and they sweep the level together
     const int64_t want = (nblk + 4 * kFuseWaves - 1) / (4 * kFuseWaves);
-    return (int)std::max<int64_t>(1, std::min<int64_t>(32 * per_cu, want));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(64, want));
 }
 
 int build_fuse_plan(const PlanLayout &lay, const RevPlanHost &rp, int fan_in, FusePlanHost *fp) {

@@ -136,8 +136,7 @@ struct FusePlanHost {
 };
 
 // Workgroup stripes of a level: about 16 blocks per wave-stripe, at most one XCD's worth of
-// resident workgroups (32 CUs x 3, or x 2 when the FP64 rows of the widest phase need more than a
-// third of a CU's 160 KB of LDS).  Fixed per degree, so the dP order is too.
+// resident workgroups (32 CUs x 2).  Fixed per degree, so the dP order is too.
 int fuse_stripes(int K, int plo, int64_t nblk);
 
 int build_fuse_plan(const PlanLayout &lay, const RevPlanHost &rp, int fan_in, FusePlanHost *fp);
