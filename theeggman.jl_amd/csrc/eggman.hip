@@ -1889,6 +1889,32 @@ static void launch_rev_fused(DevState &ds, const DevPlan &pl, const Tune &tune, 
             (const HT *)Gs, (HT *)Gc, ldg, (HT *)part, nb, ntiles);
         return;
     }
+    const int bt = pow2_at_least(nb);
+    if (bt <= 32) {
+        // batches of at most 32: the adjoint gather and the stripe sums in one launch
+        const int64_t nch = pl.lay.nst[L - 2];
+        const int64_t ntiles = (nb + bt - 1) / bt;
+        const int64_t nga = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (nch * ntiles * bt + kBlock - 1) / kBlock));
+        const int64_t n = (int64_t)nph * f.S * f.nslot * nb * kFuseWaves;
+        const int64_t ngb = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (n + kBlock - 1) / kBlock));
+        const int32_t *cent = (const int32_t *)pl.d_fent;
+        const int64_t *coff = (const int64_t *)pl.d_coff + (pl.lay.levels[L - 2] - 1);
+#define EGG_FSMALL(BTV)                                                                          \
+    k_rev_fsmall<DT, BTV><<<dim3((unsigned)(nga + ngb)), dim3(kBlock), 0, st>>>(                  \
+        nga, cent, coff, nch, (const PT *)P, ldp, (HT *)Gc, ntiles, (const int2 *)pl.d_fterms,    \
+        (const int64_t *)pl.d_ftoff + f.toff, (const int64_t *)pl.d_fprow0 + f.prow, f.S, f.nslot, \
+        pl.lay.K, f.plo, nph, (const HT *)Hc, ldh, (const HT *)Gs, ldg, (HT *)part, nb)
+        switch (bt) {
+        case 1: EGG_FSMALL(1); break;
+        case 2: EGG_FSMALL(2); break;
+        case 4: EGG_FSMALL(4); break;
+        case 8: EGG_FSMALL(8); break;
+        case 16: EGG_FSMALL(16); break;
+        case 32: EGG_FSMALL(32); break;
+        }
+#undef EGG_FSMALL
+        return;
+    }
     launch_rev_g<DT, W>(ds.caps, tune, pl, L, P, ldp, Gs, Gc, ldg, nb, st, pl.d_fent);
     if (nb >= 64) {
         const int64_t ntiles = (nb + 63) / 64;

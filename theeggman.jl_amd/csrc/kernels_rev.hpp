@@ -112,9 +112,9 @@ __global__ __launch_bounds__(WPB * 64) void k_rev_g(
 // instances), P read from global memory.  Each lane sums its instance over the child's parent
 // list in list order, as k_rev_g does, so both give the same bits.
 template <int DT, typename W, int BT>
-__global__ __launch_bounds__(kBlock) void k_rev_g_bt(
-    const W *__restrict__ cent, const int64_t *__restrict__ coff, int64_t nch,
-    const typename Elt<DT>::P *__restrict__ P, int64_t ldp,
+__device__ inline void rev_g_bt_body(
+    int64_t bid, int64_t nbid, const W *__restrict__ cent, const int64_t *__restrict__ coff,
+    int64_t nch, const typename Elt<DT>::P *__restrict__ P, int64_t ldp,
     const typename Elt<DT>::H *__restrict__ Gs, int64_t ldg,
     typename Elt<DT>::H *__restrict__ Gc, int64_t nb, int64_t ntiles) {
     using E = Elt<DT>;
@@ -122,8 +122,8 @@ __global__ __launch_bounds__(kBlock) void k_rev_g_bt(
     constexpr W mask = (W)((W(1) << bits) - 1);
     const int lane = (int)(threadIdx.x % BT);
     const int64_t ngroups = nch * ntiles;
-    const int64_t gstride = (int64_t)gridDim.x * (kBlock / BT);
-    for (int64_t g = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / BT; g < ngroups; g += gstride) {
+    const int64_t gstride = nbid * (kBlock / BT);
+    for (int64_t g = (bid * kBlock + threadIdx.x) / BT; g < ngroups; g += gstride) {
         const int64_t c = ntiles == 1 ? g : g / ntiles;
         const int64_t b = (g - c * ntiles) * BT + lane;
         if (b >= nb) continue;
@@ -150,6 +150,15 @@ __global__ __launch_bounds__(kBlock) void k_rev_g_bt(
         batch_tail<U>((int)(e1 - e), [&](auto n) { batch(e, n); });
         Gc[c * ldg + b] = acc;
     }
+}
+
+template <int DT, typename W, int BT>
+__global__ __launch_bounds__(kBlock) void k_rev_g_bt(
+    const W *__restrict__ cent, const int64_t *__restrict__ coff, int64_t nch,
+    const typename Elt<DT>::P *__restrict__ P, int64_t ldp,
+    const typename Elt<DT>::H *__restrict__ Gs, int64_t ldg,
+    typename Elt<DT>::H *__restrict__ Gc, int64_t nb, int64_t ntiles) {
+    rev_g_bt_body<DT, W, BT>(blockIdx.x, gridDim.x, cent, coff, nch, P, ldp, Gs, ldg, Gc, nb, ntiles);
 }
 
 // Partial dP sums of one level: one wave per (segment, tile), sum over the segment's
@@ -715,23 +724,58 @@ __global__ __launch_bounds__(kFuseWaves * 64) void k_rev_fused(
 
 // The same order without the phase loop, for small batches and small levels.  Four adjacent
 // threads take the kFuseWaves wave-stripes of one (phase, stripe s, slot u, instance): each sums
-// its term list (child, parent), loads issued eight at a time, and the first adds the four totals
-// in wave order (shuffles within the wave) and writes the stripe's partial row
+// its term list (child, parent) with list_sum, and the first adds the four
+// totals in wave order (shuffles within the wave) and writes the stripe's partial row
 // (rows prow0[ph] + s (K - p) + u).  G comes from the adjoint gather over the sorted lists
 // (k_rev_g / k_rev_g_bt), which sums each child's list in order.
+// Sum of the terms H[c] * G[s] of one term list (child c, parent s) for instance b, in list
+// order from zero.  Eight terms a batch; the next batch's words load while this batch's values
+// are in flight, and every batch issues the same loads (past the end, repeats of the last term,
+// whose products are not added).
 template <int DT>
-__global__ __launch_bounds__(kBlock) void k_rev_fstripe(
-    const int2 *__restrict__ terms, const int64_t *__restrict__ toff, const int64_t *__restrict__ prow0,
-    int S, int nslot, int K, int plo, int nph, const typename Elt<DT>::H *__restrict__ Hc,
-    int64_t ldh, const typename Elt<DT>::H *__restrict__ Gs, int64_t ldg,
-    typename Elt<DT>::H *__restrict__ part, int64_t nb) {
+__device__ inline typename Elt<DT>::H list_sum(const int2 *__restrict__ terms, int64_t t0, int64_t t1,
+                                               const typename Elt<DT>::H *__restrict__ Hc, int64_t ldh,
+                                               const typename Elt<DT>::H *__restrict__ Gs, int64_t ldg,
+                                               int64_t b) {
+    using E = Elt<DT>;
+    using HT = typename E::H;
+    constexpr int N = 8;
+    HT tot = E::zero();
+    const int64_t n = t1 - t0;
+    if (n <= 0) return tot;
+    int2 cs[N];
+#pragma unroll
+    for (int x = 0; x < N; x++) cs[x] = terms[t0 + min((int64_t)x, n - 1)];
+    for (int64_t b0 = 0; b0 < n; b0 += N) {
+        HT h[N], g[N];
+#pragma unroll
+        for (int x = 0; x < N; x++) {
+            h[x] = Hc[(int64_t)cs[x].x * ldh + b];
+            g[x] = Gs[(int64_t)cs[x].y * ldg + b];
+        }
+#pragma unroll
+        for (int x = 0; x < N; x++) cs[x] = terms[t0 + min(b0 + N + x, n - 1)];
+#pragma unroll
+        for (int x = 0; x < N; x++)
+            if (b0 + x < n) tot = E::muladd_hh(tot, h[x], g[x]);
+    }
+    return tot;
+}
+
+template <int DT>
+__device__ inline void fstripe_body(
+    int64_t bid, int64_t nbid, const int2 *__restrict__ terms, const int64_t *__restrict__ toff,
+    const int64_t *__restrict__ prow0, int S, int nslot, int K, int plo, int nph,
+    const typename Elt<DT>::H *__restrict__ Hc, int64_t ldh,
+    const typename Elt<DT>::H *__restrict__ Gs, int64_t ldg, typename Elt<DT>::H *__restrict__ part,
+    int64_t nb) {
     using E = Elt<DT>;
     using HT = typename E::H;
     static_assert(kFuseWaves == 4 && kBlock % 4 == 0, "a stripe's wave-stripes are four adjacent lanes");
     const int64_t V = (int64_t)S * kFuseWaves;
     const int64_t total = (int64_t)nph * S * nslot * nb * kFuseWaves;   // a multiple of 4
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t start = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = nbid * blockDim.x;
+    const int64_t start = bid * blockDim.x + threadIdx.x;
     // every thread of a wave runs the same number of iterations (the shuffles need all four)
     for (int64_t i0 = start - (threadIdx.x & 63); i0 < total; i0 += stride) {
         const int64_t i = i0 + (threadIdx.x & 63);
@@ -747,22 +791,7 @@ __global__ __launch_bounds__(kBlock) void k_rev_fstripe(
         HT tot = E::zero();
         if (in && u < nrow) {
             const int64_t li = ((int64_t)ph * nslot + u) * V + s * kFuseWaves + w;
-            const int64_t t0 = toff[li], t1 = toff[li + 1];
-            auto batch = [&](int64_t t, auto n) {
-                constexpr int N = decltype(n)::value;
-                HT h[N], g[N];
-#pragma unroll
-                for (int x = 0; x < N; x++) {
-                    const int2 cs = terms[t + x];
-                    h[x] = Hc[(int64_t)cs.x * ldh + b];
-                    g[x] = Gs[(int64_t)cs.y * ldg + b];
-                }
-#pragma unroll
-                for (int x = 0; x < N; x++) tot = E::muladd_hh(tot, h[x], g[x]);
-            };
-            int64_t t = t0;
-            for (; t + 8 <= t1; t += 8) batch(t, std::integral_constant<int, 8>{});
-            batch_tail<8>((int)(t1 - t), [&](auto n) { batch(t, n); });
+            tot = list_sum<DT>(terms, toff[li], toff[li + 1], Hc, ldh, Gs, ldg, b);
         }
         // the four totals in wave order
         const int lane = (int)(threadIdx.x & 63), l0 = lane & ~3;
@@ -773,9 +802,38 @@ __global__ __launch_bounds__(kBlock) void k_rev_fstripe(
     }
 }
 
+template <int DT>
+__global__ __launch_bounds__(kBlock) void k_rev_fstripe(
+    const int2 *__restrict__ terms, const int64_t *__restrict__ toff, const int64_t *__restrict__ prow0,
+    int S, int nslot, int K, int plo, int nph, const typename Elt<DT>::H *__restrict__ Hc,
+    int64_t ldh, const typename Elt<DT>::H *__restrict__ Gs, int64_t ldg,
+    typename Elt<DT>::H *__restrict__ part, int64_t nb) {
+    fstripe_body<DT>(blockIdx.x, gridDim.x, terms, toff, prow0, S, nslot, K, plo, nph, Hc, ldh, Gs, ldg,
+                     part, nb);
+}
+
+// The small-batch reverse step of one level in one launch: workgroups [0, nga) run the adjoint
+// gather over the sorted lists (BT lanes per child), the others the stripe sums; both read only
+// G of the parents' level, so they are independent.
+template <int DT, int BT>
+__global__ __launch_bounds__(kBlock) void k_rev_fsmall(
+    int64_t nga, const int32_t *__restrict__ cent, const int64_t *__restrict__ coff, int64_t nch,
+    const typename Elt<DT>::P *__restrict__ P, int64_t ldp, typename Elt<DT>::H *__restrict__ Gc,
+    int64_t ntiles, const int2 *__restrict__ terms, const int64_t *__restrict__ toff,
+    const int64_t *__restrict__ prow0, int S, int nslot, int K, int plo, int nph,
+    const typename Elt<DT>::H *__restrict__ Hc, int64_t ldh,
+    const typename Elt<DT>::H *__restrict__ Gs, int64_t ldg, typename Elt<DT>::H *__restrict__ part,
+    int64_t nb) {
+    if ((int64_t)blockIdx.x < nga)
+        rev_g_bt_body<DT, int32_t, BT>(blockIdx.x, nga, cent, coff, nch, P, ldp, Gs, ldg, Gc, nb, ntiles);
+    else
+        fstripe_body<DT>(blockIdx.x - nga, (int64_t)gridDim.x - nga, terms, toff, prow0, S, nslot, K,
+                         plo, nph, Hc, ldh, Gs, ldg, part, nb);
+}
+
 // k_rev_fstripe for batches of 64 or more: one workgroup per (phase, stripe s, slot u,
 // 64-instance tile), wave w summing wave-stripe 4s + w's term list with lanes over instances
-// (uniform terms, whole-row loads), and the four totals added in wave order through LDS.
+// (uniform terms, whole-row loads; list_sum), and the four totals added in wave order through LDS.
 template <int DT>
 __global__ __launch_bounds__(kFuseWaves * 64) void k_rev_fstripe_w(
     const int2 *__restrict__ terms, const int64_t *__restrict__ toff, const int64_t *__restrict__ prow0,
@@ -800,23 +858,7 @@ __global__ __launch_bounds__(kFuseWaves * 64) void k_rev_fstripe_w(
         const int64_t b = tile * 64 + lane;
         const int64_t bl = b < nb ? b : nb - 1;
         const int64_t li = ((int64_t)ph * nslot + u) * V + s * kFuseWaves + w;
-        const int64_t t0 = toff[li], t1 = toff[li + 1];
-        HT tot = E::zero();
-        auto batch = [&](int64_t t, auto n) {
-            constexpr int N = decltype(n)::value;
-            HT h[N], gv[N];
-#pragma unroll
-            for (int x = 0; x < N; x++) {
-                const int2 cs = terms[t + x];
-                h[x] = Hc[(int64_t)cs.x * ldh + bl];
-                gv[x] = Gs[(int64_t)cs.y * ldg + bl];
-            }
-#pragma unroll
-            for (int x = 0; x < N; x++) tot = E::muladd_hh(tot, h[x], gv[x]);
-        };
-        int64_t t = t0;
-        for (; t + 8 <= t1; t += 8) batch(t, std::integral_constant<int, 8>{});
-        batch_tail<8>((int)(t1 - t), [&](auto n) { batch(t, n); });
+        const HT tot = list_sum<DT>(terms, toff[li], toff[li + 1], Hc, ldh, Gs, ldg, bl);
         tw[w][lane] = tot;
         __syncthreads();
         if (w == 0) {
