@@ -56,6 +56,46 @@ def test_exact_gradient_is_the_deletion_hafnian(N):
                for i in range(N) for j in range(i + 1, N)) * 2 == N * vals[0]
 
 
+@pytest.mark.parametrize("T,N,B", [(np.float64, 20, 70), (np.float64, 28, 1024),
+                                   (np.float32, 16, 70), (np.complex64, 12, 70)])
+def test_fused_gradient_is_exact_on_integer_matrices(T, N, B):
+    # 0/1 adjacency matrices (Gaussian integers a + bi, a, b in {0, 1}, for complex): every value
+    # and partial sum of the walk and of the reverse sweep is an integer far below 2^53 (2^24 for
+    # single precision), so the floating-point gradient of the fused reverse step must equal the
+    # exact one bit for bit whatever its summation order -- through the fused kernel (passes of
+    # 64 and more) and through the small-batch kernels (passes of 1 and 7)
+    note(f"fused grad exact {np.dtype(T)} N={N} B={B}")
+    if np.dtype(T).kind == "c":
+        As = np.stack([synth.gnp_adjacency(N, 71, b) + 1j * synth.gnp_adjacency(N, 72, b)
+                       for b in range(B)])
+    else:
+        As = np.stack([synth.gnp_adjacency(N, 71, b) for b in range(B)]).astype(np.float64)
+    P = te._gather_pairs_batch(T, As, None, N)
+    plan = te._dp_plan(N)
+    runs = [te._haf_dp_backend_grad(BE, P, plan)]
+    for mb in ((64, 7) if B <= 100 else (512, 64)):
+        runs.append(te._haf_dp_backend_grad(BE, P, plan, max_batch=mb))
+    if np.dtype(T).kind == "c":
+        # the deletion hafnians of a few instances from the oracle's complex128 walk (exact here)
+        want = np.zeros((3, N * (N - 1) // 2), np.complex128)
+        for b in range(3):
+            q = 0
+            for i in range(N):
+                for j in range(i + 1, N):
+                    want[b, q] = deletion(As[b], i, j, o.C64)
+                    q += 1
+        for v, g in runs:
+            assert np.array_equal(np.asarray(g)[:3].astype(np.complex128), want)
+        return
+    Pi = te._gather_pairs_batch(np.int64, As.astype(np.int64), None, N)
+    vi, gi = te._haf_dp_backend_grad(BE, Pi, plan)
+    want_v = np.array([float(x) for x in vi], T)
+    want_g = np.array([[float(x) for x in row] for row in gi], T)
+    for v, g in runs:
+        assert np.asarray(v).tobytes() == want_v.tobytes()
+        assert np.asarray(g).tobytes() == want_g.tobytes()
+
+
 @pytest.mark.parametrize("T,N", [(np.float64, 16), (np.float64, 20), (np.complex128, 16)])
 def test_float_gradient_matches_deletion_identity(T, N):
     note(f"grad {np.dtype(T)} N={N}")
