@@ -56,20 +56,31 @@ def test_exact_gradient_is_the_deletion_hafnian(N):
                for i in range(N) for j in range(i + 1, N)) * 2 == N * vals[0]
 
 
-@pytest.mark.parametrize("T,N,B", [(np.float64, 20, 70), (np.float64, 28, 1024),
-                                   (np.float32, 16, 70), (np.complex64, 12, 70)])
-def test_fused_gradient_is_exact_on_integer_matrices(T, N, B):
+def sparse_adjacency(N, config, b, sparsity):
+    # 0/1 adjacency of a G(N, 2^-sparsity) graph: the AND of independent G(N, 1/2) graphs
+    A = synth.gnp_adjacency(N, config, b)
+    for x in range(1, sparsity):
+        A = A & synth.gnp_adjacency(N, config + 100 * x, b)
+    return A
+
+
+@pytest.mark.parametrize("T,N,B,sparsity", [(np.float64, 26, 70, 1), (np.float64, 28, 1024, 1),
+                                            (np.float32, 26, 70, 2), (np.complex64, 26, 70, 4)])
+def test_fused_gradient_is_exact_on_integer_matrices(T, N, B, sparsity):
     # 0/1 adjacency matrices (Gaussian integers a + bi, a, b in {0, 1}, for complex): every value
-    # and partial sum of the walk and of the reverse sweep is an integer far below 2^53 (2^24 for
-    # single precision), so the floating-point gradient of the fused reverse step must equal the
-    # exact one bit for bit whatever its summation order -- through the fused kernel (passes of
-    # 64 and more) and through the small-batch kernels (passes of 1 and 7)
+    # and partial sum of the walk and of the reverse sweep is a non-negative integer (a Gaussian
+    # integer) of at most the hafnian of a subgraph, far below 2^53 for G(N, 1/2), and below 2^24
+    # for the sparser graphs of the single-precision cases (G(26, 1/4) real, G(26, 1/16) per
+    # component complex: about 10^5 and 10^3), so the
+    # floating-point gradient of the fused reverse step (degrees 26 and up) must equal the exact
+    # one bit for bit whatever its summation order -- through the fused kernel (passes of 64 and
+    # more) and through the small-batch kernels (passes of 1 and 7)
     note(f"fused grad exact {np.dtype(T)} N={N} B={B}")
     if np.dtype(T).kind == "c":
-        As = np.stack([synth.gnp_adjacency(N, 71, b) + 1j * synth.gnp_adjacency(N, 72, b)
+        As = np.stack([sparse_adjacency(N, 71, b, sparsity) + 1j * sparse_adjacency(N, 72, b, sparsity)
                        for b in range(B)])
     else:
-        As = np.stack([synth.gnp_adjacency(N, 71, b) for b in range(B)]).astype(np.float64)
+        As = np.stack([sparse_adjacency(N, 71, b, sparsity) for b in range(B)]).astype(np.float64)
     P = te._gather_pairs_batch(T, As, None, N)
     plan = te._dp_plan(N)
     runs = [te._haf_dp_backend_grad(BE, P, plan)]
@@ -152,7 +163,7 @@ def test_config4_batched_gradient_n24():
     _ = rng
 
 
-@pytest.mark.parametrize("N", [16, 24])
+@pytest.mark.parametrize("N", [16, 24, 28])
 def test_gradient_is_deterministic_and_chunk_independent(N):
     B = 70
     note(f"grad chunks N={N}")

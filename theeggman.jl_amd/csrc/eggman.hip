@@ -1581,6 +1581,7 @@ constexpr int kParWaves = 8;
 constexpr int kParSub = 16;
 constexpr int kParTaskLen = kParWaves * kParSub;
 constexpr int kRedFanIn = 64;
+constexpr int kFuseMinK = 26;   // smallest degree of the fused reverse step (use_fused)
 
 static int upload_vec(void **dst, const void *src, size_t bytes, size_t *acc) {
     hipError_t e = hipMalloc(dst, std::max<size_t>(bytes, 16));
@@ -1636,7 +1637,7 @@ static int ensure_rev(DevPlan &pl) {
     pl.nred = (int64_t)rp.red.size();
     pl.lvl_seg = rp.lvl_seg;
     pl.nseg = (int64_t)rp.segs.size();
-    if (lay.index_bytes == 4) {
+    if (lay.index_bytes == 4 && lay.K >= kFuseMinK) {
         FusePlanHost fp;
         EGG_TRY(build_fuse_plan(lay, rp, kRedFanIn, &fp));
         std::vector<ParTaskDev> ft(fp.tasks.size());
@@ -1673,9 +1674,15 @@ static int ensure_rev(DevPlan &pl) {
 // Device bytes one instance needs for forward + reverse: the kept levels of H and the segment
 // region (every level, without checkpoints), two levels of G, the dP partials (segment sums, or
 // task rows plus stage-one rows), and its P and dP columns.
-// The fused child-major reverse step serves the Int32 layout and elements of at most 8 bytes
-// (its per-wave totals live in registers); the others keep the parent-major or segment dP.
-static bool use_fused(const DevPlan &pl, int dtype) { return pl.fuse && elt_hbytes(dtype) <= 8; }
+// The fused child-major reverse step serves degrees from kFuseMinK up on the Int32 layout, for
+// elements of at most 8 bytes (its LDS rows and registers are sized for them); the others keep
+// the parent-major or segment dP.  Below kFuseMinK the levels are small, most of them would take
+// the stripe-list kernels, and the parent-major dP is faster (N=22, B=4096: 3.0 against 4.5 ms;
+// N=24 even; N=26, B=2048: 11.5 against 10.1 ms).  The choice depends on the degree and element
+// type only, so each degree's gradient has one summation order for every pass size.
+static bool use_fused(const DevPlan &pl, int dtype) {
+    return pl.fuse && pl.lay.K >= kFuseMinK && elt_hbytes(dtype) <= 8;
+}
 
 static size_t grad_bytes_per_instance(const DevPlan &pl, int dtype, bool ckpt) {
     const size_t hb = elt_hbytes(dtype), pb = elt_pbytes(dtype);
