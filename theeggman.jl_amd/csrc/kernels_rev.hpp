@@ -127,27 +127,30 @@ __device__ inline void rev_g_bt_body(
         const int64_t c = ntiles == 1 ? g : g / ntiles;
         const int64_t b = (g - c * ntiles) * BT + lane;
         if (b >= nb) continue;
-        const int64_t e0 = coff[c], e1 = coff[c + 1];
+        const int64_t e0 = coff[c], n = coff[c + 1] - e0;
         typename E::H acc = E::zero();
-        constexpr int U = 8;   // gathers issued together before they are summed in order
-        auto batch = [&](int64_t e, auto n) {
-            constexpr int N = decltype(n)::value;
-            W t[N];
-            typename E::P pv[N];
-            typename E::H gv[N];
+        // eight entries a batch, summed in list order; the next batch's entry words load while
+        // this batch's values are in flight (past the end, repeats of the last entry, not added)
+        constexpr int U = 8;
+        if (n > 0) {
+            W t[U];
 #pragma unroll
-            for (int u = 0; u < N; u++) t[u] = cent[e + u];
+            for (int u = 0; u < U; u++) t[u] = cent[e0 + min((int64_t)u, n - 1)];
+            for (int64_t b0 = 0; b0 < n; b0 += U) {
+                typename E::P pv[U];
+                typename E::H gv[U];
 #pragma unroll
-            for (int u = 0; u < N; u++) {
-                pv[u] = P[((int64_t)(t[u] & mask) - 1) * ldp + b];
-                gv[u] = Gs[(int64_t)(t[u] >> bits) * ldg + b];
+                for (int u = 0; u < U; u++) {
+                    pv[u] = P[((int64_t)(t[u] & mask) - 1) * ldp + b];
+                    gv[u] = Gs[(int64_t)(t[u] >> bits) * ldg + b];
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) t[u] = cent[e0 + min(b0 + U + u, n - 1)];
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                    if (b0 + u < n) acc = E::muladd(acc, pv[u], gv[u]);
             }
-#pragma unroll
-            for (int u = 0; u < N; u++) acc = E::muladd(acc, pv[u], gv[u]);
-        };
-        int64_t e = e0;
-        for (; e + U <= e1; e += U) batch(e, std::integral_constant<int, U>{});
-        batch_tail<U>((int)(e1 - e), [&](auto n) { batch(e, n); });
+        }
         Gc[c * ldg + b] = acc;
     }
 }
