@@ -296,11 +296,23 @@ int64_t eggman_batch_bytes(int K, int dtype, int64_t B);
  * chosen to minimise kept_rows + seg_rows, where the other levels are recomputed segment by
  * segment in a region of seg_rows state rows.  kept[L] = 1 for a kept level (kept[0] unused,
  * nkept >= K/2 + 1).  Rows count state values per instance.  New: the reference has no
- * derivative; the north-star reverse sweep recomputes levels from checkpoints above a memory
- * threshold (EGGMAN_TUNE_CKPT forces it).
+ * derivative; the reverse sweep recomputes levels from checkpoints above a memory threshold
+ * (EGGMAN_TUNE_CKPT forces it).
  */
 int eggman_grad_levels(int K, int checkpointed, int64_t *kept_rows, int64_t *seg_rows,
                        uint8_t *kept, int64_t nkept);
+
+/*
+ * Builds, on the host, the index tables of the gradient's fused reverse step for degree K
+ * (Int32 layout, 4 <= K <= 32) from the host plan and checks them against it (host-only; no
+ * GPU needed): every child's parent list, sorted by pair, holds for each phase p < min(child)
+ * exactly the parents child + {p, j}, j ascending, j not in the child; every stripe term list
+ * holds its (phase, slot, wave-stripe)'s terms in (block, rank) order; the partial-row table
+ * covers every (level, phase, stripe).  Counts the list entries and terms it checked.  Returns
+ * EGGMAN_OK, or EGGMAN_EPLAN naming the first inconsistency.  New: the reference has no
+ * derivative; this is the self-check of the tables behind eggman_dp_forward_grad.
+ */
+int eggman_fuse_validate(int K, int64_t *entries_checked, int64_t *terms_checked);
 
 #ifdef __cplusplus
 }

@@ -284,3 +284,15 @@ def test_exact_bound_refuses_what_128_bits_cannot_hold():
     # extreme entries do not break the bound's arithmetic
     rc, _ = _exact_bound([np.full((4, 4), np.iinfo(np.int64).min, np.int64)])
     assert rc == _lib.EOVERFLOW
+
+
+@pytest.mark.parametrize("K", [4, 6, 12, 16, 20, 26, 28])
+def test_fused_reverse_tables_follow_the_plan(K):
+    # the gradient's fused reverse step reads each child's parents sorted by pair, phase by phase,
+    # and sums dP over stripe term lists; the library rebuilds those tables from its host plan and
+    # checks every entry and term against the plan's own transitions (a state's element set is its
+    # child's plus the transition's pair), so every transition appears once in each
+    ne, nt = ctypes.c_int64(), ctypes.c_int64()
+    _lib.check(_lib.lib().eggman_fuse_validate(K, ctypes.byref(ne), ctypes.byref(nt)), "fuse_validate")
+    ntrans = te._dp_counts(K)[1]
+    assert ne.value == ntrans and nt.value == ntrans

@@ -3051,6 +3051,19 @@ int eggman_grad_levels(int K, int checkpointed, int64_t *kept_rows, int64_t *seg
     return EGGMAN_OK;
 }
 
+int eggman_fuse_validate(int K, int64_t *entries_checked, int64_t *terms_checked) {
+    if (K < 4 || K > 32) return fail(EGGMAN_EINVAL, "the fused reverse step's tables need 4 <= K <= 32, got %d", K);
+    PlanLayout lay;
+    EGG_TRY(plan_layout(K, 4, &lay));
+    std::vector<int32_t> starts((size_t)lay.nstates + 1), trans((size_t)lay.ntrans), levels(lay.nlevels);
+    EGG_TRY(build_plan_host(lay, starts.data(), trans.data(), levels.data(), 0));
+    RevPlanHost rp;
+    EGG_TRY(build_rev_plan(lay, trans.data(), kRevSegLen, kParTaskLen, kRedFanIn, &rp));
+    FusePlanHost fp;
+    EGG_TRY(build_fuse_plan(lay, rp, kRedFanIn, &fp));
+    return validate_fuse_plan(lay, trans.data(), rp, fp, entries_checked, terms_checked);
+}
+
 }  // extern "C"
 #endif
 

@@ -559,4 +559,98 @@ int build_fuse_plan(const PlanLayout &lay, const RevPlanHost &rp, int fan_in, Fu
     return EGGMAN_OK;
 }
 
+int validate_fuse_plan(const PlanLayout &lay, const int32_t *trans, const RevPlanHost &rp,
+                       const FusePlanHost &fp, int64_t *entries, int64_t *terms) {
+    const int K = lay.K;
+    const int32_t mask = (1 << lay.bits) - 1;
+    std::vector<int> qi(lay.npairs + 1), qj(lay.npairs + 1);
+    for (int i = 1; i < K; i++)
+        for (int j = i + 1; j <= K; j++) {
+            qi[pair_index(i, j, K)] = i;
+            qj[pair_index(i, j, K)] = j;
+        }
+    // element masks against the plan: a state's mask is its child's plus {p, j}
+    for (int L = 2; L <= K / 2 + 1; L++) {
+        const int k = lay.size[L - 1] - 1;
+        const int64_t s0 = lay.levels[L - 1] - 1;
+        for (int64_t r = 0; r < lay.nst[L - 1]; r++)
+            for (int v = 0; v < k; v++) {
+                const int32_t w = trans[lay.tstart[L - 1] + r * k + v];
+                const int64_t c = (int64_t)(w >> lay.bits) - 1;
+                const int q = w & mask;
+                if (fp.cmask[s0 + r] != (fp.cmask[c] | (1u << (qi[q] - 1)) | (1u << (qj[q] - 1))))
+                    return fail(EGGMAN_EPLAN, "element mask of state %lld disagrees with its transition %d",
+                                (long long)(s0 + r), v);
+            }
+    }
+    int64_t ne = 0, nt = 0, rows = 0;
+    for (int L = 2; L <= K / 2 + 1; L++) {
+        const FuseLevel &f = fp.lv[L];
+        const int64_t pfirst = lay.levels[L - 1] - 1;
+        if (f.cfirst != lay.levels[L - 2] - 1 || f.nch != lay.nst[L - 2] || f.plo != lay.lo[L - 1] ||
+            f.S < 1 || f.nblk != (f.nch + kFuseT - 1) / kFuseT)
+            return fail(EGGMAN_EPLAN, "level %d: fused layout disagrees with the plan", L);
+        // each child's sorted list: phases p = plo .. min(c) - 1, each the n_p parents c + {p, j}
+        for (int64_t c = 0; c < f.nch; c++) {
+            const uint32_t cm = fp.cmask[f.cfirst + c];
+            const int32_t *lst = fp.fent.data() + f.ent0 + fp.fco[f.cfirst + c];
+            const int64_t len = rp.coff[f.cfirst + c + 1] - rp.coff[f.cfirst + c];
+            int64_t off = 0;
+            for (int p = f.plo; p <= f.phi; p++) {
+                if ((cm & ((1u << p) - 1u)) != 0) break;
+                const int np = K - p - f.csize;
+                int lastj = p;
+                for (int u = 0; u < np; u++, ne++) {
+                    const int32_t e = lst[off + u];
+                    const int q = e & mask;
+                    if (off + u >= len || qi[q] != p || qj[q] <= lastj || ((cm >> (qj[q] - 1)) & 1u) ||
+                        fp.cmask[pfirst + (e >> lay.bits)] != (cm | (1u << (p - 1)) | (1u << (qj[q] - 1))))
+                        return fail(EGGMAN_EPLAN, "level %d child %lld: phase %d entry %d is wrong", L,
+                                    (long long)c, p, u);
+                    lastj = qj[q];
+                }
+                off += np;
+            }
+            if (off != len)
+                return fail(EGGMAN_EPLAN, "level %d child %lld: list of %lld entries, phases cover %lld", L,
+                            (long long)c, (long long)len, (long long)off);
+        }
+        // term lists in (block, rank) order, each term a transition of its pair and stripe
+        const int64_t V = (int64_t)f.S * kFuseWaves;
+        for (int p = f.plo; p <= f.phi; p++) {
+            if (fp.prow0[f.prow + (p - f.plo)] != rows)
+                return fail(EGGMAN_EPLAN, "level %d phase %d: partial rows start at %lld, expected %lld", L,
+                            p, (long long)fp.prow0[f.prow + (p - f.plo)], (long long)rows);
+            rows += (int64_t)f.S * (K - p);
+            for (int u = 0; u < f.nslot; u++)
+                for (int64_t v = 0; v < V; v++) {
+                    const int64_t li = f.toff + ((int64_t)(p - f.plo) * f.nslot + u) * V + v;
+                    int64_t lastc = -1;
+                    for (int64_t t = fp.toff[li]; t < fp.toff[li + 1]; t++, nt++) {
+                        const int32_t c = fp.terms[2 * t], sr = fp.terms[2 * t + 1];
+                        const int j = p + 1 + u;
+                        const uint32_t cm = (c >= 0 && c < f.nch) ? fp.cmask[f.cfirst + c] : 0xffffffffu;
+                        if (c <= lastc || c >= f.nch || (c / kFuseT) % V != v || (cm & ((1u << p) - 1u)) ||
+                            j > K || ((cm >> (j - 1)) & 1u) || sr < 0 || sr >= lay.nst[L - 1] ||
+                            fp.cmask[pfirst + sr] != (cm | (1u << (p - 1)) | (1u << (j - 1))))
+                            return fail(EGGMAN_EPLAN, "level %d phase %d slot %d stripe %lld: term %lld is wrong",
+                                        L, p, u, (long long)v, (long long)t);
+                        lastc = c;
+                    }
+                }
+        }
+    }
+    if (ne != lay.ntrans || nt != lay.ntrans || rows != fp.part_rows)
+        return fail(EGGMAN_EPLAN, "fused tables hold %lld entries and %lld terms for %lld transitions, "
+                    "%lld partial rows for %lld", (long long)ne, (long long)nt, (long long)lay.ntrans,
+                    (long long)rows, (long long)fp.part_rows);
+    // every task's rows lie in the partial buffer
+    for (const ParTask &tk : fp.tasks)
+        if (tk.row0 < 0 || tk.row0 + (K - tk.p) > fp.part_rows)
+            return fail(EGGMAN_EPLAN, "a partial-row task lies outside the buffer");
+    if (entries) *entries = ne;
+    if (terms) *terms = nt;
+    return EGGMAN_OK;
+}
+
 }  // namespace eggman

@@ -141,6 +141,10 @@ int fuse_stripes(int K, int plo, int64_t nblk);
 
 int build_fuse_plan(const PlanLayout &lay, const RevPlanHost &rp, int fan_in, FusePlanHost *fp);
 
+// Checks the tables of build_fuse_plan against the plan's transitions (see eggman_fuse_validate).
+int validate_fuse_plan(const PlanLayout &lay, const int32_t *trans, const RevPlanHost &rp,
+                       const FusePlanHost &fp, int64_t *entries, int64_t *terms);
+
 // Where the gradient keeps the forward values of levels 1 .. K/2 (level L's values are read by
 // the reverse step at L + 1; the top level's value is the result).  A kept level stays resident
 // in the kept region at row `off[L]`; the other levels form segments (maximal runs of levels

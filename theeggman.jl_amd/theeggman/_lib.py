@@ -66,6 +66,7 @@ SIGNATURES = {
     "eggman_cache_clear": (_int, []),
     "eggman_batch_bytes": (_i64, [_int, _int, _i64]),
     "eggman_grad_levels": (_int, [_int, _int, _i64p, _i64p, _vp, _i64]),
+    "eggman_fuse_validate": (_int, [_int, _i64p, _i64p]),
 }
 
 _lib = None
