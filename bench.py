@@ -60,6 +60,23 @@ def compulsory_bytes(nst, K, B, chunk, tbytes, ibytes, small=4096):
     return reads + writes
 
 
+def grad_compulsory_bytes(nst, K, B, passes, tbytes, ibytes):
+    """HBM bytes one forward + gradient call cannot avoid, in `passes` passes: the forward part
+    writes every state value once and reads each once as a child; the reverse sweep reads every
+    parent adjoint once and every child value once, and writes every child adjoint once; the
+    plan's transition words are read once per pass by each part; P is read by each part and dP
+    written once."""
+    states = sum(int(n) for n in nst)
+    top = len(nst)
+    ntrans = sum(int(nst[L - 1]) * (2 * (L - 1) - 1) for L in range(2, top + 1))
+    npairs = K * (K - 1) // 2
+    below_top = states - int(nst[-1])
+    reads = B * below_top * tbytes + B * (states - 1) * tbytes + B * below_top * tbytes
+    reads += 2 * passes * ntrans * ibytes + 2 * B * npairs * tbytes
+    writes = B * (states - 1) * tbytes + B * below_top * tbytes + B * npairs * tbytes
+    return reads, writes
+
+
 def compulsory_split(nst, K, B, chunk, tbytes, ibytes, small=4096):
     """(reads, writes) of compulsory_bytes."""
     passes, lb, lt = walk_split(nst, B, chunk, small)
@@ -633,6 +650,18 @@ def main():
         msg, _ = time_device(hip, stream, grad_step, gsteps, 1)
         result["fwd_grad"] = {"ms_per_step": msg, "hafnians_with_gradient_per_s": B / (msg / 1e3),
                               "forward_transitions_per_s": B * nt / (msg / 1e3), "steps": gsteps}
+        # compulsory traffic of the call (passes of 512 instances at N=28; the library raises the
+        # forward's tuned pass to at least 512 for the gradient)
+        gpass = max(512, B if B < 512 else 512)
+        grd, gwr = grad_compulsory_bytes(nst, K, B, -(-B // gpass), 8, plan.index_bytes)
+        result["fwd_grad"]["roofline"] = {
+            "bound": "hbm", "alg_bytes": "compulsory: forward part writes each state once and reads "
+            "each once as a child; reverse reads each parent adjoint and child value once and writes "
+            "each child adjoint once; plan words once per pass per part; P, dP",
+            "compulsory_bytes": grd + gwr, "achieved": (grd + gwr) / (msg / 1e3) / 1e9,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": (grd + gwr) / (msg / 1e3) / 1e9 / HBM_PEAK_GBS,
+            "counters": "profiles/grad_trace_counters_n28_b1024.md"}
         del dPd
 
     if not args.no_plain:
