@@ -341,3 +341,33 @@ def test_checkpointed_gradient_is_bit_identical(N, T, I):
     else:
         assert v0.tobytes() == v1.tobytes() == v2.tobytes() == v3.tobytes()
         assert np.asarray(g0).tobytes() == np.asarray(g2).tobytes() == np.asarray(g3).tobytes()
+
+
+def test_fused_gradient_with_64bit_level_offsets():
+    # one pass of 4352 instances puts the largest N=28 level's rows (125970 states) beyond 4 GB of
+    # their base, so the fused reverse step addresses them with 64-bit offsets; its gradient must
+    # be the same bits as in passes of 512 instances (32-bit offsets)
+    N, B = 28, 4352
+    plan = te._dp_plan(N)
+    assert int(np.diff(plan.levels).max()) * B * 8 >= 1 << 32
+    note(f"fused grad 64-bit rows N={N} B={B}")
+    As = np.stack([synth.sym_uniform(N, 73, b) for b in range(B)])
+    P = te._gather_pairs_batch(np.float64, As, None, N)
+    code, ib = te.tuning.dtype_code(np.float64), plan.index_bytes
+    # the entry B's bucket resolves to now (its own, else the nearest tuned bucket below), so that
+    # only the pass size changes and the entry can be put back as it was
+    prev = next((e for bb in (B, 4096, 2048, 1024, 512, 256, 128, 64)
+                 if (e := te.tuning.tune_get(N, np.float64, bb)) is not None), None)
+    assert prev is not None, "no tuned entry for N=28 Float64"
+
+    def entry(chunk):
+        _lib.check(_lib.lib().eggman_tune_set(N, ib, code, B, prev["bt"], chunk, prev["flags"],
+                                              prev["occ"]), "tune_set")
+    entry(B)
+    try:
+        v1, g1 = te._haf_dp_backend_grad(BE, P, plan)
+    finally:
+        entry(prev["chunk"])
+    v2, g2 = te._haf_dp_backend_grad(BE, P, plan, max_batch=512)
+    assert np.asarray(v1).tobytes() == np.asarray(v2).tobytes()
+    assert np.asarray(g1).tobytes() == np.asarray(g2).tobytes()
