@@ -515,6 +515,9 @@ def main():
     chunk = min((tuned or {}).get("chunk") or B, B)
     nst = np.diff(np.asarray(plan.levels, dtype=np.int64))
     launches = level_launches(nst, B, chunk)
+    # the library walks passes two at a time on two streams when there are four or more (and the
+    # values are at most 8 bytes), so two level launches are in flight at once
+    in_flight = 2 if -(-B // chunk) >= 4 else 1
     tl = tuned or {}
     head = ("k_level_fwd_lds_vec" if tl.get("vec") else
             "k_level_fwd_lds_narrow" if tl.get("narrow") else "k_level_fwd_lds") \
@@ -551,6 +554,11 @@ def main():
                      "alg_bytes": "compulsory: each state value written once and read once, "
                                   "plan words once per walk of a level, pair values once",
                      "avg_launch_ms": ms / launches,
+                     "launches_in_flight": in_flight,
+                     "launch_time_note": "avg_launch_ms is the step time per launch; with two "
+                                         "passes in flight each launch lasts about twice that "
+                                         "(the kernel trace's average duration), and the device "
+                                         "rate is the same",
                      "per_transition_model": {
                          "bytes_per_step": model_bytes,
                          "gbs": model_bytes / (ms / 1e3) / 1e9,
@@ -583,10 +591,11 @@ def main():
         rl["frac_of_copy"] = rl["achieved"] / copy_gbs
         # the practical ceiling of the level kernel's gathers: random reads of whole 128-byte
         # lines (8 lanes x 16 bytes, as the kernel reads a child row) from a working set the size
-        # of a pass's two level buffers, measured on this GPU
+        # of the level buffers of the passes in flight, measured on this GPU
         import ctypes
         probe = {}
-        for name, ws in (("l2_resident", 2 << 20), ("pass_buffers", 2 * int(nst.max()) * chunk * 8)):
+        for name, ws in (("l2_resident", 2 << 20),
+                         ("pass_buffers", in_flight * 2 * int(nst.max()) * chunk * 8)):
             lps = ctypes.c_double()
             _lib.check(_lib.lib().eggman_gather_rate(ws, ctypes.byref(lps)), "gather_rate")
             probe[name] = {"working_set_bytes": ws, "line_gbs": lps.value * 128 / 1e9}
@@ -594,7 +603,8 @@ def main():
         if prof and prof.get("l1_to_l2_read_requests_per_launch"):
             # the kernel's L1-to-L2 line requests per second (counter run) against the probe's
             # rates blended by the kernel's L2 hit rate: its hits at the L2-resident rate, its
-            # misses at the rate of a working set the size of a pass's two level buffers
+            # misses at the rate of a working set the size of the level buffers of the passes in
+            # flight
             kl = prof["l1_to_l2_read_requests_per_launch"] * 128 / (rl["avg_launch_ms"] / 1e3) / 1e9
             h = prof["l2"]["hit_rate"]
             ceil = 1.0 / (h / probe["l2_resident"]["line_gbs"] + (1 - h) / probe["pass_buffers"]["line_gbs"])

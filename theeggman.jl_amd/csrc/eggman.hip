@@ -229,6 +229,53 @@ static int ensure(DevBuf &b, size_t bytes) {
     return EGGMAN_OK;
 }
 
+// The side streams (default priority) and their fork/join events, created on first use.
+static int ensure_side(DevState &ds) {
+    if (ds.ev_fork) return EGGMAN_OK;
+    for (int i = 0; i < DevState::kSide; i++) {
+        EGG_HIP(hipStreamCreateWithFlags(&ds.side[i], hipStreamNonBlocking));
+        EGG_HIP(hipEventCreateWithFlags(&ds.ev_join[i], hipEventDisableTiming));
+    }
+    EGG_HIP(hipEventCreateWithFlags(&ds.ev_fork, hipEventDisableTiming));
+    return EGGMAN_OK;
+}
+
+// Work of one call split over `st` and ns - 1 side streams: fork makes the side streams wait for
+// what st holds so far, join makes st wait for everything the side streams were given.
+static int fork_side(DevState &ds, hipStream_t st, int ns) {
+    if (ns <= 1) return EGGMAN_OK;
+    EGG_TRY(ensure_side(ds));
+    EGG_HIP(hipEventRecord(ds.ev_fork, st));
+    for (int i = 0; i < ns - 1; i++) EGG_HIP(hipStreamWaitEvent(ds.side[i], ds.ev_fork, 0));
+    return EGGMAN_OK;
+}
+static int join_side(DevState &ds, hipStream_t st, int ns) {
+    for (int i = 0; i < ns - 1; i++) {
+        EGG_HIP(hipEventRecord(ds.ev_join[i], ds.side[i]));
+        EGG_HIP(hipStreamWaitEvent(st, ds.ev_join[i], 0));
+    }
+    return EGGMAN_OK;
+}
+// Stream i of such a split: st itself, then the side streams.
+static hipStream_t split_stream(const DevState &ds, hipStream_t st, int i) {
+    return i == 0 ? st : ds.side[i - 1];
+}
+
+// Passes (or a gradient pass's sub-passes) run two at a time, alternating between the call's
+// stream and a side stream, when there are at least four of them and the state values are at most
+// 8 bytes: the short latency-bound launches at either end of one pass's level run (a few µs each
+// on a few thousand states) then overlap the other pass's large levels.  Passes write disjoint
+// columns or buffers, so the values are the same bits.  Measured (forward, ms, one stream
+// against two): FP64 N=28 B=1024 3.22 -> 3.07, B=4096 13.2 -> 12.3, B=256 0.879 -> 0.840; FP32
+// N=28 1.73 -> 1.63; complex64 N=28 3.38 -> 3.06; FP64 N=32 B=256 7.43 -> 7.20; N=24 B=4096 1.79
+// -> 1.78.  Two passes only (N=24 B=1024: 0.472 -> 0.483) and 16-byte values (complex128 N=28:
+// 6.32 -> 6.86, their two working sets contend in L2) stay on one stream; three or four streams
+// were slower than two (N=28: 3.31 and 3.44 ms).
+constexpr int kPassStreams = 2;
+static int pass_streams(int dtype, int64_t npass) {
+    return npass >= 4 && elt_hbytes(dtype) <= 8 ? kPassStreams : 1;
+}
+
 static int find_plan(uint64_t handle, DevPlan **pl) {
     auto it = g_plans.find(handle);
     if (it == g_plans.end())
@@ -1088,6 +1135,9 @@ static int forward_levels(DevState &ds, const DevPlan &pl, const void *P, int64_
         launch_level_bt<DT, W>(tune.bt, pl, L, Pc, ldp, Hc, ldh, Ho, ldo, nb, st);
     };
     if (tune.pairs) EGG_TRY(ensure_items(pl));
+    // passes two at a time (pass_streams), each stream with its own pair of level buffers; not
+    // in caller-provided memory or under graph capture
+    const int ns = ws || tune.graph ? 1 : pass_streams(DT, (B + chunk - 1) / chunk);
     char *hmem, *wmem = nullptr;
     if (ws) {
         if (w.h_bytes + w.w_bytes > ws->bytes)
@@ -1096,7 +1146,7 @@ static int forward_levels(DevState &ds, const DevPlan &pl, const void *P, int64_
         hmem = ws->p;
         wmem = ws->p + w.h_bytes;
     } else {
-        EGG_TRY(ensure(ds.H, w.h_bytes));
+        EGG_TRY(ensure(ds.H, w.h_bytes * ns));
         hmem = (char *)ds.H.p;
         if (w.w_bytes > 0) {
             EGG_TRY(ensure(ds.Hw, w.w_bytes));
@@ -1117,11 +1167,16 @@ static int forward_levels(DevState &ds, const DevPlan &pl, const void *P, int64_
         for (int L = 2; L <= lb; L++)
             launch(L, (const char *)P, wb[(L - 1) & 1], ldw, wb[L & 1], ldw, B);
     }
-    for (int64_t b0 = 0; b0 < B; b0 += chunk) {
+    const hipStream_t st0 = st;
+    EGG_TRY(fork_side(ds, st0, ns));
+    for (int64_t b0 = 0, ip = 0; b0 < B; b0 += chunk, ip++) {
         const int64_t nb = std::min(chunk, B - b0);
         const int64_t ld = pad ? pad_rows(nb) : nb;
         const char *Pc = (const char *)P + (size_t)b0 * pb;
         char *outc = (char *)out + (size_t)b0 * hb;
+        st = split_stream(ds, st0, (int)(ip % ns));
+        buf[0] = hmem + (size_t)(ip % ns) * w.h_bytes;
+        buf[1] = buf[0] + (size_t)w.rows * w.ldmax * hb;
         if (fb > 1)
             launch_fused_range<DT, W>(ds.caps, pl, 2, fb, Pc, ldp, nullptr, 0,
                                       fb == top ? (void *)outc : (void *)buf[fb & 1], ld, nb, st);
@@ -1145,6 +1200,8 @@ static int forward_levels(DevState &ds, const DevPlan &pl, const void *P, int64_
         if (ft <= top)
             launch_fused_range<DT, W>(ds.caps, pl, ft, top, Pc, ldp, buf[(ft - 1) & 1], ld, outc, 1, nb, st);
     }
+    st = st0;
+    EGG_TRY(join_side(ds, st, ns));
     for (int L = lt; L <= top; L++)
         launch(L, (const char *)P, wt[(L - 1) & 1], ldw, L == top ? out : (void *)wt[L & 1], ldw, B);
     EGG_HIP(hipGetLastError());
@@ -1488,16 +1545,8 @@ int forward_mixed_dt(DevState &ds, const std::vector<DevPlan *> &pls,
     }
     if (!solo.empty()) {
         EGG_TRY(ensure(ds.Hm, std::max<size_t>(total, 256)));
-        if (!ds.ev_fork) {
-            EGG_HIP(hipEventCreateWithFlags(&ds.ev_fork, hipEventDisableTiming));
-            for (int i = 0; i < DevState::kSide; i++) {
-                EGG_HIP(hipStreamCreateWithFlags(&ds.side[i], hipStreamNonBlocking));
-                EGG_HIP(hipEventCreateWithFlags(&ds.ev_join[i], hipEventDisableTiming));
-            }
-        }
-        EGG_HIP(hipEventRecord(ds.ev_fork, st));
         const int nside = (int)std::min<size_t>(solo.size(), DevState::kSide);
-        for (int i = 0; i < nside; i++) EGG_HIP(hipStreamWaitEvent(ds.side[i], ds.ev_fork, 0));
+        EGG_TRY(fork_side(ds, st, nside + 1));
         size_t off = 0;
         int rc = EGGMAN_OK;
         for (size_t i = 0; i < solo.size() && rc == EGGMAN_OK; i++) {
@@ -1507,10 +1556,7 @@ int forward_mixed_dt(DevState &ds, const std::vector<DevPlan *> &pls,
                                ds.side[i % DevState::kSide], tunes[i], &ws);
             off += span[i];
         }
-        for (int i = 0; i < nside; i++) {
-            EGG_HIP(hipEventRecord(ds.ev_join[i], ds.side[i]));
-            EGG_HIP(hipStreamWaitEvent(st, ds.ev_join[i], 0));
-        }
+        EGG_TRY(join_side(ds, st, nside + 1));
         EGG_TRY(rc);
     }
     if (spl.empty()) return EGGMAN_OK;
@@ -1996,16 +2042,26 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
                   launch_level_lds<DT, W, 1>(ds.caps, tune, pl, L, Ps, ldp, Hc, ldh, Ho, ldh, ns, st)))
                 launch_level_bt<DT, W>(pow2_at_least(ns), pl, L, Ps, ldp, Hc, ldh, Ho, ldh, ns, st);
         };
-        auto forward_range = [&](int L0, int L1) {
+        // (sub-passes two at a time as the forward's passes, pass_streams: they write disjoint
+        // columns)
+        auto forward_range = [&](int L0, int L1) -> int {
             int lb = L0, lt = L1;
             while (lb <= lt && lay.nst[lb - 1] <= kSmallLevel) launch_cols(lb++, 0, nb);
             while (lt >= lb && lay.nst[lt - 1] <= kSmallLevel) lt--;
-            for (int64_t s0 = 0; s0 < nb && lb <= lt; s0 += sub)
+            const int ns = lb <= lt ? pass_streams(DT, (nb + sub - 1) / sub) : 1;
+            const hipStream_t st0 = st;
+            EGG_TRY(fork_side(ds, st0, ns));
+            for (int64_t s0 = 0, i = 0; s0 < nb && lb <= lt; s0 += sub, i++) {
+                st = split_stream(ds, st0, (int)(i % ns));
                 for (int L = lb; L <= lt; L++) launch_cols(L, s0, std::min(sub, nb - s0));
+            }
+            st = st0;
+            EGG_TRY(join_side(ds, st, ns));
             for (int L = lt + 1; L <= L1; L++) launch_cols(L, 0, nb);
+            return EGGMAN_OK;
         };
         k_fill_one<DT><<<dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st>>>(level_rows(1), nb);
-        forward_range(2, top);
+        EGG_TRY(forward_range(2, top));
         // the segment the forward left in the segment region: the run of levels below top
         int seg_lo = top, seg_hi = top - 1;
         if (!gl.kept[top - 1]) {
@@ -2021,7 +2077,7 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
                 seg_hi = L - 1;
                 seg_lo = L - 1;
                 while (!gl.kept[seg_lo - 1]) seg_lo--;
-                forward_range(seg_lo, seg_hi);
+                EGG_TRY(forward_range(seg_lo, seg_hi));
             }
             if (fused) {
                 launch_rev_fused<DT, W>(ds, pl, tune, L, Pc, ldp, level_rows(L - 1), ldh, G[L & 1],
