@@ -272,6 +272,12 @@ static hipStream_t split_stream(const DevState &ds, hipStream_t st, int i) {
 // 6.32 -> 6.86, their two working sets contend in L2) stay on one stream; three or four streams
 // were slower than two (N=28: 3.31 and 3.44 ms).
 constexpr int kPassStreams = 2;
+// Gradient passes in flight at once (forward_grad_levels): N=28 FP64, forward + gradient per
+// call, against one pass at a time with its forward sub-passes split over two streams:
+// B=1024 11.79 -> 11.65 ms (11.82 -> 11.49 in another session), B=2048 23.5 -> 22.9,
+// N=26 B=2048 9.99 -> 9.80; splitting each pass's sub-passes too was no better (11.79, 23.3,
+// 10.07).
+constexpr int kGradStreams = 2;
 static int pass_streams(int dtype, int64_t npass) {
     return npass >= 4 && elt_hbytes(dtype) <= 8 ? kPassStreams : 1;
 }
@@ -2007,18 +2013,36 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
     const GradLevels &gl = tune.ckpt ? pl.gl_ckpt : pl.gl_full;
     // forward level rows: padded to whole tile rows for the 16-byte-load kernel
     const bool pad = tune.lds && tune.vec;
-    EGG_TRY(ensure(ds.Hg, (size_t)std::max<int64_t>(gl.kept_rows + gl.seg_rows, 1) *
-                              (pad ? pad_rows(chunk) : chunk) * hb));
     const int64_t ldmax = pad ? pad_rows(chunk) : chunk;
-    EGG_TRY(ensure(ds.G, (size_t)2 * lay.max_level * ldmax * hb));
-    EGG_TRY(ensure(ds.part, (size_t)std::max<int64_t>(fused ? pl.fpart_rows + pl.fnred
+    // one pass's buffers: every kept level (and a segment), two adjoint levels, the dP partials
+    const size_t hg_bytes = (size_t)std::max<int64_t>(gl.kept_rows + gl.seg_rows, 1) * ldmax * hb;
+    const size_t g_bytes = (size_t)2 * lay.max_level * ldmax * hb;
+    const size_t p_bytes = (size_t)std::max<int64_t>(fused ? pl.fpart_rows + pl.fnred
                                                            : par ? pl.part_rows + pl.nred : pl.nseg, 1) *
-                                chunk * hb));
-    HT *H = (HT *)ds.Hg.p;
-    HT *G[2] = {(HT *)ds.G.p, (HT *)ds.G.p + (size_t)lay.max_level * ldmax};
-    HT *part = (HT *)ds.part.p;
+                           chunk * hb;
+    // two passes in flight (the call's stream and a side stream, each with its own buffers) when
+    // there are two or more, the values are at most 8 bytes and a second set of buffers fits in
+    // 80% of the free memory; the sub-passes of a pass then run one at a time
+    int np = 1;
+    if ((B + chunk - 1) / chunk >= 2 && elt_hbytes(DT) <= 8) {
+        size_t free_b = 0, total_b = 0;
+        EGG_HIP(hipMemGetInfo(&free_b, &total_b));
+        const size_t have = ds.Hg.bytes + ds.G.bytes + ds.part.bytes;
+        if (2 * (hg_bytes + g_bytes + p_bytes) <= have + (size_t)(0.8 * (double)free_b)) np = kGradStreams;
+    }
+    EGG_TRY(ensure(ds.Hg, hg_bytes * np));
+    EGG_TRY(ensure(ds.G, g_bytes * np));
+    EGG_TRY(ensure(ds.part, p_bytes * np));
     const int top = lay.K / 2 + 1;
-    for (int64_t b0 = 0; b0 < B; b0 += chunk) {
+    const hipStream_t stc = st;
+    EGG_TRY(fork_side(ds, stc, np));
+    for (int64_t b0 = 0, ip = 0; b0 < B; b0 += chunk, ip++) {
+        const int si = (int)(ip % np);
+        st = split_stream(ds, stc, si);
+        HT *H = (HT *)((char *)ds.Hg.p + si * hg_bytes);
+        HT *G[2] = {(HT *)((char *)ds.G.p + si * g_bytes),
+                    (HT *)((char *)ds.G.p + si * g_bytes) + (size_t)lay.max_level * ldmax};
+        HT *part = (HT *)((char *)ds.part.p + si * p_bytes);
         const int64_t nb = std::min(chunk, B - b0);
         const char *Pc = (const char *)P + (size_t)b0 * pb;
         HT *outc = (HT *)out + b0;
@@ -2048,7 +2072,7 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
             int lb = L0, lt = L1;
             while (lb <= lt && lay.nst[lb - 1] <= kSmallLevel) launch_cols(lb++, 0, nb);
             while (lt >= lb && lay.nst[lt - 1] <= kSmallLevel) lt--;
-            const int ns = lb <= lt ? pass_streams(DT, (nb + sub - 1) / sub) : 1;
+            const int ns = lb <= lt && np == 1 ? pass_streams(DT, (nb + sub - 1) / sub) : 1;
             const hipStream_t st0 = st;
             EGG_TRY(fork_side(ds, st0, ns));
             for (int64_t s0 = 0, i = 0; s0 < nb && lb <= lt; s0 += sub, i++) {
@@ -2126,6 +2150,8 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
             k_rev_dp_reduce<DT><<<dim3((unsigned)blocks), dim3(256), 0, st>>>(
                 (const int64_t *)pl.d_qoff, (const int32_t *)pl.d_qseg, part, dPc, lddp, lay.npairs, nb);
     }
+    st = stc;
+    EGG_TRY(join_side(ds, st, np));
     EGG_HIP(hipGetLastError());
     return EGGMAN_OK;
 }
