@@ -131,7 +131,10 @@ int eggman_dp_forward(uint64_t plan, int dtype, const void *P_host, int64_t B, i
  * Device-resident form of eggman_dp_forward: P_dev is [pair][ldp] in device memory (instance b
  * of pair q at q*ldp + b, ldp >= B), out_dev receives B results.  Asynchronous on `stream`
  * (a hipStream_t; NULL = the library's stream).  No host transfer happens.  New: the
- * reference only exposes the host form.
+ * reference only exposes the host form.  A call of four or more passes runs them two at a time,
+ * on `stream` and on a side stream of the library's that waits for `stream` where the call's
+ * passes begin and that `stream` waits for before the call's last launches: the call stays
+ * ordered on `stream` like a single launch sequence.
  */
 int eggman_dp_forward_device(uint64_t plan, int dtype, const void *P_dev, int64_t ldp,
                              int64_t B, void *out_dev, void *stream);
@@ -201,7 +204,8 @@ int eggman_dp_forward_grad(uint64_t plan, int dtype, const void *P_host, int64_t
                            int64_t npairs, int64_t max_batch, void *out_host, void *dP_host);
 
 /* Device-resident form of eggman_dp_forward_grad: dP_dev rows are lddp elements apart.
- * Asynchronous on `stream` (NULL = the library's stream). */
+ * Asynchronous on `stream` (NULL = the library's stream); passes may run two at a time on
+ * `stream` and a side stream, joined back into `stream` as in eggman_dp_forward_device. */
 int eggman_dp_forward_grad_device(uint64_t plan, int dtype, const void *P_dev, int64_t ldp,
                                   int64_t B, void *out_dev, void *dP_dev, int64_t lddp,
                                   void *stream);
