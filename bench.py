@@ -611,6 +611,10 @@ def main():
             rl["l1_to_l2_line_gbs"] = kl
             rl["gather_ceiling_gbs"] = ceil
             rl["frac_of_gather_ceiling"] = kl / ceil
+            # the blend is an estimate (the probe's misses are random over the whole working set,
+            # the kernel's follow the colex sweep, and two launches overlap); the probe's
+            # all-L2-hit rate bounds any line-request stream from above
+            rl["frac_of_l2_resident_line_rate"] = kl / probe["l2_resident"]["line_gbs"]
     if rl["traffic"]:
         rl["traffic_gbs"] = rl["traffic"] / (rl["avg_launch_ms"] / 1e3) / 1e9
         rl["traffic_frac"] = rl["traffic_gbs"] / HBM_PEAK_GBS
