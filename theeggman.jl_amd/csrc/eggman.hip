@@ -146,6 +146,10 @@ struct DevState {
     static constexpr int kSide = 4;
     hipStream_t side[kSide] = {};
     hipEvent_t ev_fork = nullptr, ev_join[kSide] = {};
+    // second streams of the walks that run on side streams (their passes two at a time), and the
+    // event that links one stream to another
+    hipStream_t sub[kSide] = {};
+    hipEvent_t ev_link = nullptr;
     void *scratch_dev = nullptr;     // this device's address of the no-allocation scratch array
     // host entry points: a two-slot pipeline of pinned host buffers and device buffers, with an
     // upload and a download stream beside the compute stream (run_pipe)
@@ -234,9 +238,19 @@ static int ensure_side(DevState &ds) {
     if (ds.ev_fork) return EGGMAN_OK;
     for (int i = 0; i < DevState::kSide; i++) {
         EGG_HIP(hipStreamCreateWithFlags(&ds.side[i], hipStreamNonBlocking));
+        EGG_HIP(hipStreamCreateWithFlags(&ds.sub[i], hipStreamNonBlocking));
         EGG_HIP(hipEventCreateWithFlags(&ds.ev_join[i], hipEventDisableTiming));
     }
+    EGG_HIP(hipEventCreateWithFlags(&ds.ev_link, hipEventDisableTiming));
     EGG_HIP(hipEventCreateWithFlags(&ds.ev_fork, hipEventDisableTiming));
+    return EGGMAN_OK;
+}
+
+// `waiter` waits for what `signaler` holds so far.
+static int link_streams(DevState &ds, hipStream_t waiter, hipStream_t signaler) {
+    EGG_TRY(ensure_side(ds));
+    EGG_HIP(hipEventRecord(ds.ev_link, signaler));
+    EGG_HIP(hipStreamWaitEvent(waiter, ds.ev_link, 0));
     return EGGMAN_OK;
 }
 
@@ -1101,11 +1115,14 @@ static void walk_plan(const DevCaps &caps, const PlanLayout &lay, const Tune &tu
     w->w_bytes = (size_t)2 * (w->wrows_b + w->wrows_t) * w->ldw * hb;
 }
 
-// Caller-provided state memory (the no-allocation entry point): forward_levels then carves its
-// level buffers from it instead of the library's grow-only buffers, and allocates nothing.
+// Caller-provided state memory (the no-allocation entry point, the concurrent walks of a mixed
+// call): forward_levels then carves its level buffers from it instead of the library's grow-only
+// buffers, and allocates nothing.  `aux`: a second stream the walk may run every other pass on,
+// when the memory holds a second pair of level buffers (null: one stream, nothing created).
 struct WorkSpan {
     char *p = nullptr;
     size_t bytes = 0;
+    hipStream_t aux = nullptr;
 };
 
 // Walk all levels for instances [0, B) of P (rows ldp apart) into out, `chunk` instances per
@@ -1142,15 +1159,18 @@ static int forward_levels(DevState &ds, const DevPlan &pl, const void *P, int64_
     };
     if (tune.pairs) EGG_TRY(ensure_items(pl));
     // passes two at a time (pass_streams), each stream with its own pair of level buffers; not
-    // in caller-provided memory or under graph capture
-    const int ns = ws || tune.graph ? 1 : pass_streams(DT, (B + chunk - 1) / chunk);
+    // under graph capture, and in caller-provided memory only with a second stream and room for
+    // the second pair
+    int ns = tune.graph ? 1 : pass_streams(DT, (B + chunk - 1) / chunk);
+    if (ws && (!ws->aux || (size_t)ns * w.h_bytes + w.w_bytes > ws->bytes)) ns = 1;
+    const hipStream_t aux = ns > 1 && ws ? ws->aux : nullptr;
     char *hmem, *wmem = nullptr;
     if (ws) {
         if (w.h_bytes + w.w_bytes > ws->bytes)
             return fail(EGGMAN_ENOMEM, "the walk needs %.1f MiB of state memory, %.1f MiB are provided",
                         (w.h_bytes + w.w_bytes) / 1048576.0, ws->bytes / 1048576.0);
         hmem = ws->p;
-        wmem = ws->p + w.h_bytes;
+        wmem = ws->p + (size_t)ns * w.h_bytes;
     } else {
         EGG_TRY(ensure(ds.H, w.h_bytes * ns));
         hmem = (char *)ds.H.p;
@@ -1174,13 +1194,14 @@ static int forward_levels(DevState &ds, const DevPlan &pl, const void *P, int64_
             launch(L, (const char *)P, wb[(L - 1) & 1], ldw, wb[L & 1], ldw, B);
     }
     const hipStream_t st0 = st;
-    EGG_TRY(fork_side(ds, st0, ns));
+    if (aux) EGG_TRY(link_streams(ds, aux, st0));
+    else EGG_TRY(fork_side(ds, st0, ns));
     for (int64_t b0 = 0, ip = 0; b0 < B; b0 += chunk, ip++) {
         const int64_t nb = std::min(chunk, B - b0);
         const int64_t ld = pad ? pad_rows(nb) : nb;
         const char *Pc = (const char *)P + (size_t)b0 * pb;
         char *outc = (char *)out + (size_t)b0 * hb;
-        st = split_stream(ds, st0, (int)(ip % ns));
+        st = aux && ip % ns ? aux : split_stream(ds, st0, (int)(ip % ns));
         buf[0] = hmem + (size_t)(ip % ns) * w.h_bytes;
         buf[1] = buf[0] + (size_t)w.rows * w.ldmax * hb;
         if (fb > 1)
@@ -1207,7 +1228,8 @@ static int forward_levels(DevState &ds, const DevPlan &pl, const void *P, int64_
             launch_fused_range<DT, W>(ds.caps, pl, ft, top, Pc, ldp, buf[(ft - 1) & 1], ld, outc, 1, nb, st);
     }
     st = st0;
-    EGG_TRY(join_side(ds, st, ns));
+    if (aux) EGG_TRY(link_streams(ds, st0, aux));
+    else EGG_TRY(join_side(ds, st, ns));
     for (int L = lt; L <= top; L++)
         launch(L, (const char *)P, wt[(L - 1) & 1], ldw, L == top ? out : (void *)wt[L & 1], ldw, B);
     EGG_HIP(hipGetLastError());
@@ -1544,7 +1566,9 @@ int forward_mixed_dt(DevState &ds, const std::vector<DevPlan *> &pls,
         if (t.pairs) EGG_TRY(ensure_items(*pls[g]));
         WalkPlan w;
         walk_plan(ds.caps, pls[g]->lay, t, counts[g], DT, &w);
-        const size_t bytes = (w.h_bytes + w.w_bytes + 255) & ~(size_t)255;
+        // room for a second pair of level buffers when the walk runs its passes two at a time
+        const int ns = w.fused ? 1 : pass_streams(DT, (counts[g] + w.chunk - 1) / w.chunk);
+        const size_t bytes = ((size_t)ns * w.h_bytes + w.w_bytes + 255) & ~(size_t)255;
         tunes.push_back(t);
         span.push_back(bytes);
         total += bytes;
@@ -1557,7 +1581,7 @@ int forward_mixed_dt(DevState &ds, const std::vector<DevPlan *> &pls,
         int rc = EGGMAN_OK;
         for (size_t i = 0; i < solo.size() && rc == EGGMAN_OK; i++) {
             const size_t g = solo[i];
-            const WorkSpan ws{(char *)ds.Hm.p + off, span[i]};
+            const WorkSpan ws{(char *)ds.Hm.p + off, span[i], ds.sub[i % DevState::kSide]};
             rc = forward_eager(DT, ds, *pls[g], Pv[g], counts[g], counts[g], outv[g],
                                ds.side[i % DevState::kSide], tunes[i], &ws);
             off += span[i];
