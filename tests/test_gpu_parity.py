@@ -241,3 +241,31 @@ def test_untuned_defaults_bit_exact(T, N, B):
     As = stack(N, B, 15, "complex" if np.dtype(T).kind == "c" else "normal").astype(T)
     got = te.hafnian(As, backend=BE, check_symmetric=False)
     assert got.tobytes() == oracle_vals(As, T).tobytes()
+
+
+@pytest.mark.parametrize("T", [np.float64, np.float32, np.complex64, np.complex128])
+def test_passes_in_flight_bit_identical(T):
+    # four or more passes of at most 8-byte values run two at a time on two streams, each with
+    # its own level buffers (a ragged last pass included); one pass, and 16-byte values (which stay
+    # on one stream), are the references
+    N, B = 20, 1000
+    note(f"passes in flight N={N} B={B} {np.dtype(T)}")
+    kind = "complex" if np.iscomplexobj(np.zeros(1, T)) else "uniform"
+    As = stack(N, B, 71, kind).astype(T)
+    P = te._gather_pairs_batch(T, As, None, N)
+    plan = te._dp_plan(N)
+    code = te.tuning.dtype_code(T)
+    prev = te.tuning.tune_get(N, T, B)
+    flags = te.tuning.LDS | te.tuning.VEC
+    try:
+        _lib.check(_lib.lib().eggman_tune_set(N, plan.index_bytes, code, B, 64, 64, flags, 2), "tune_set")
+        split = te._haf_dp_backend(BE, P, plan)        # 16 passes
+        _lib.check(_lib.lib().eggman_tune_set(N, plan.index_bytes, code, B, 64, 0, flags, 2), "tune_set")
+        whole = te._haf_dp_backend(BE, P, plan)        # one pass
+    finally:
+        if prev is not None:
+            _lib.check(_lib.lib().eggman_tune_set(N, plan.index_bytes, code, B, prev["bt"], prev["chunk"],
+                                                  prev["flags"], prev["occ"]), "tune_set")
+    assert split.tobytes() == whole.tobytes()
+    if T in DT:
+        assert split.tobytes() == oracle_vals(As, T).tobytes()
