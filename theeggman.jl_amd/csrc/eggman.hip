@@ -277,8 +277,9 @@ static hipStream_t split_stream(const DevState &ds, hipStream_t st, int i) {
 
 // Passes (or a gradient pass's sub-passes) run two at a time, alternating between the call's
 // stream and a side stream, when there are at least four of them and the state values are at most
-// 8 bytes: the short latency-bound launches at either end of one pass's level run (a few µs each
-// on a few thousand states) then overlap the other pass's large levels.  Passes write disjoint
+// 8 bytes: each launch's ramp-up and drain, and the short latency-bound launches of a pass's
+// small levels (a few µs each on a few thousand states), then overlap the other pass's work.  The
+// two passes run in step, level by level, reading the same plan words; passes write disjoint
 // columns or buffers, so the values are the same bits.  Measured (forward, ms, one stream
 // against two): FP64 N=28 B=1024 3.22 -> 3.07, B=4096 13.2 -> 12.3, B=256 0.879 -> 0.840; FP32
 // N=28 1.73 -> 1.63; complex64 N=28 3.38 -> 3.06; FP64 N=32 B=256 7.43 -> 7.20; N=24 B=4096 1.79
