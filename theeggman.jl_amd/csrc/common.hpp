@@ -33,7 +33,9 @@ int fail(int code, const char *fmt, ...);
 // Every muladd below is written as a separate multiply and add: the sources are compiled with
 // fp-contract off, so no fused multiply-add is formed and a state's sum matches the reference's
 // _dp_states! (src/dp.jl:344-363) bit for bit when accumulated in plan order.  Complex multiply
-// is (ac - bd, ad + bc), Julia's `*` for Complex.
+// is (ac - bd, ad + bc), Julia's `*` for Complex.  The product is rounded before it meets the
+// accumulator, so muladd_hh(acc, a, b) == add_h(acc, mul_hh(a, b)) bit for bit: products may be
+// formed in any order as long as they are added in the list's.
 
 // The tail of a gather loop taken in batches of U: `batch(n)` with n an integral_constant equal to
 // rem (0 < rem < U), so the last rem loads are issued together like a full batch's instead of one
@@ -55,6 +57,7 @@ template <> struct Elt<EGGMAN_F64> {
     __device__ static inline H one() { return 1.0; }
     __device__ static inline H muladd(H acc, P p, H h) { return acc + p * h; }
     __device__ static inline H muladd_hh(H acc, H a, H b) { return acc + a * b; }
+    __device__ static inline H mul_hh(H a, H b) { return a * b; }
     __device__ static inline H add_h(H a, H b) { return a + b; }
 };
 template <> struct Elt<EGGMAN_F32> {
@@ -64,6 +67,7 @@ template <> struct Elt<EGGMAN_F32> {
     __device__ static inline H one() { return 1.0f; }
     __device__ static inline H muladd(H acc, P p, H h) { return acc + p * h; }
     __device__ static inline H muladd_hh(H acc, H a, H b) { return acc + a * b; }
+    __device__ static inline H mul_hh(H a, H b) { return a * b; }
     __device__ static inline H add_h(H a, H b) { return a + b; }
 };
 template <> struct Elt<EGGMAN_C64> {
@@ -77,6 +81,9 @@ template <> struct Elt<EGGMAN_C64> {
         return make_double2(acc.x + re, acc.y + im);
     }
     __device__ static inline H muladd_hh(H acc, H a, H b) { return muladd(acc, a, b); }
+    __device__ static inline H mul_hh(H a, H b) {
+        return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    }
     __device__ static inline H add_h(H a, H b) { return make_double2(a.x + b.x, a.y + b.y); }
 };
 template <> struct Elt<EGGMAN_C32> {
@@ -90,6 +97,9 @@ template <> struct Elt<EGGMAN_C32> {
         return make_float2(acc.x + re, acc.y + im);
     }
     __device__ static inline H muladd_hh(H acc, H a, H b) { return muladd(acc, a, b); }
+    __device__ static inline H mul_hh(H a, H b) {
+        return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    }
     __device__ static inline H add_h(H a, H b) { return make_float2(a.x + b.x, a.y + b.y); }
 };
 // Exact integer walk: states are 128-bit integers modulo 2^128, pair values are int64.  The
@@ -116,6 +126,7 @@ template <> struct Elt<EGGMAN_I128> {
         return make_ulonglong2(lo, a.y + b.y + (lo < a.x ? 1ull : 0ull));
     }
     __device__ static inline H muladd_hh(H acc, H a, H b) { return add(acc, mul(a, b)); }
+    __device__ static inline H mul_hh(H a, H b) { return mul(a, b); }
     __device__ static inline H add_h(H a, H b) { return add(a, b); }
 };
 

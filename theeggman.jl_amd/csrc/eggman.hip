@@ -1979,12 +1979,11 @@ static void launch_rev_fused(DevState &ds, const DevPlan &pl, const Tune &tune, 
         const int64_t nch = pl.lay.nst[L - 2];
         const int64_t ntiles = (nb + bt - 1) / bt;
         const int64_t nga = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (nch * ntiles * bt + kBlock - 1) / kBlock));
-        const int64_t n = (int64_t)nph * f.S * f.nslot * nb * kFuseWaves;
-        const int64_t ngb = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (n + kBlock - 1) / kBlock));
+        const int64_t ngb = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (int64_t)nph * f.S * f.nslot));
         const int32_t *cent = (const int32_t *)pl.d_fent;
         const int64_t *coff = (const int64_t *)pl.d_coff + (pl.lay.levels[L - 2] - 1);
 #define EGG_FSMALL(BTV)                                                                          \
-    k_rev_fsmall<DT, BTV><<<dim3((unsigned)(nga + ngb)), dim3(kBlock), 0, st>>>(                  \
+    k_rev_fsmall<DT, BTV><<<dim3((unsigned)(nga + ngb)), dim3(kBlock), fstripe_lds_bytes<DT, BTV>(), st>>>( \
         nga, cent, coff, nch, (const PT *)P, ldp, (HT *)Gc, ntiles, (const int2 *)pl.d_fterms,    \
         (const int64_t *)pl.d_ftoff + f.toff, (const int64_t *)pl.d_fprow0 + f.prow, f.S, f.nslot, \
         pl.lay.K, f.plo, nph, (const HT *)Hc, ldh, (const HT *)Gs, ldg, (HT *)part, nb)

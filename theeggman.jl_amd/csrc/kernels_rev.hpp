@@ -805,6 +805,81 @@ __device__ inline void fstripe_body(
     }
 }
 
+// Orders a wave's LDS writes before its later LDS reads (and reads before later writes): the LDS
+// executes one wave's operations in order, so this only keeps the compiler from moving them.
+__device__ inline void wave_lds_order() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// The stripe sums for batches of at most 32 instances (k_rev_fsmall), in the order of
+// fstripe_body: one workgroup per (phase, stripe s, slot u), wave w summing wave-stripe
+// 4s + w's term list for every instance.  A chunk of CT terms is multiplied out with the lanes over
+// (term, instance) pairs into LDS, then each instance's lane adds the chunk's products in list
+// order; wave 0 adds the four totals in wave order.  The sums are list_sum's to the bit (a product
+// is rounded before it meets the total), without its chain of one dependent round trip per eight
+// terms, which on the largest levels' lists (up to 288 terms at N=28) set the kernel's time.
+template <int DT, int BT>
+__device__ inline void fstripe_lds_body(
+    int64_t bid, int64_t nbid, const int2 *__restrict__ terms, const int64_t *__restrict__ toff,
+    const int64_t *__restrict__ prow0, int S, int nslot, int K, int plo, int nph,
+    const typename Elt<DT>::H *__restrict__ Hc, int64_t ldh,
+    const typename Elt<DT>::H *__restrict__ Gs, int64_t ldg, typename Elt<DT>::H *__restrict__ part,
+    int64_t nb, typename Elt<DT>::H *lds) {
+    using E = Elt<DT>;
+    using HT = typename E::H;
+    static_assert(kBlock == kFuseWaves * 64 && BT <= 32, "one wave per wave-stripe, BT <= 32");
+    constexpr int CT = BT <= 16 ? 64 : 32;     // terms per chunk: at most 1024 products per wave
+    constexpr int R = (CT * BT + 63) / 64;     // (term, instance) pairs per lane and chunk
+    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    HT *buf = lds + (size_t)w * CT * BT;
+    HT *tw = lds + (size_t)kFuseWaves * CT * BT;   // the four waves' totals, [wave][instance]
+    const int64_t V = (int64_t)S * kFuseWaves;
+    const int64_t nitems = (int64_t)nph * S * nslot;
+    for (int64_t it = bid; it < nitems; it += nbid) {
+        const int u = (int)(it % nslot);
+        const int64_t ps = it / nslot;
+        const int64_t s = ps % S;
+        const int ph = (int)(ps / S);
+        const int nrow = K - plo - ph;
+        if (u >= nrow) continue;   // the same for the whole workgroup
+        const int64_t li = ((int64_t)ph * nslot + u) * V + s * kFuseWaves + w;
+        const int64_t t0 = toff[li], t1 = toff[li + 1];
+        HT acc = E::zero();
+        for (int64_t c0 = t0; c0 < t1; c0 += CT) {
+            const int n = (int)min((int64_t)CT, t1 - c0);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int idx = lane + 64 * r, t = idx / BT, b = idx % BT;
+                if (t < n && b < nb) {
+                    const int2 cs = terms[c0 + t];
+                    buf[t * BT + b] = E::mul_hh(Hc[(int64_t)cs.x * ldh + b], Gs[(int64_t)cs.y * ldg + b]);
+                }
+            }
+            wave_lds_order();
+            if (lane < nb)
+                for (int t = 0; t < n; t++) acc = E::add_h(acc, buf[t * BT + lane]);
+            wave_lds_order();
+        }
+        if (lane < BT) tw[w * BT + lane] = acc;
+        __syncthreads();
+        if (w == 0 && lane < nb) {
+            HT sum = tw[lane];
+#pragma unroll
+            for (int x = 1; x < kFuseWaves; x++) sum = E::add_h(sum, tw[x * BT + lane]);
+            part[(prow0[ph] + s * nrow + u) * nb + lane] = sum;
+        }
+        __syncthreads();
+    }
+}
+
+// LDS bytes of fstripe_lds_body<DT, BT>.
+template <int DT, int BT>
+constexpr size_t fstripe_lds_bytes() {
+    return (size_t)kFuseWaves * ((BT <= 16 ? 64 : 32) + 1) * BT * sizeof(typename Elt<DT>::H);
+}
+
 template <int DT>
 __global__ __launch_bounds__(kBlock) void k_rev_fstripe(
     const int2 *__restrict__ terms, const int64_t *__restrict__ toff, const int64_t *__restrict__ prow0,
@@ -816,8 +891,9 @@ __global__ __launch_bounds__(kBlock) void k_rev_fstripe(
 }
 
 // The small-batch reverse step of one level in one launch: workgroups [0, nga) run the adjoint
-// gather over the sorted lists (BT lanes per child), the others the stripe sums; both read only
-// G of the parents' level, so they are independent.
+// gather over the sorted lists (BT lanes per child), the others the stripe sums (one workgroup per
+// (phase, stripe, slot), fstripe_lds_body, with fstripe_lds_bytes<DT, BT>() of dynamic LDS); both
+// read only G of the parents' level, so they are independent.
 template <int DT, int BT>
 __global__ __launch_bounds__(kBlock) void k_rev_fsmall(
     int64_t nga, const int32_t *__restrict__ cent, const int64_t *__restrict__ coff, int64_t nch,
@@ -827,11 +903,13 @@ __global__ __launch_bounds__(kBlock) void k_rev_fsmall(
     const typename Elt<DT>::H *__restrict__ Hc, int64_t ldh,
     const typename Elt<DT>::H *__restrict__ Gs, int64_t ldg, typename Elt<DT>::H *__restrict__ part,
     int64_t nb) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
     if ((int64_t)blockIdx.x < nga)
         rev_g_bt_body<DT, int32_t, BT>(blockIdx.x, nga, cent, coff, nch, P, ldp, Gs, ldg, Gc, nb, ntiles);
     else
-        fstripe_body<DT>(blockIdx.x - nga, (int64_t)gridDim.x - nga, terms, toff, prow0, S, nslot, K,
-                         plo, nph, Hc, ldh, Gs, ldg, part, nb);
+        fstripe_lds_body<DT, BT>(blockIdx.x - nga, (int64_t)gridDim.x - nga, terms, toff, prow0, S,
+                                 nslot, K, plo, nph, Hc, ldh, Gs, ldg, part, nb,
+                                 reinterpret_cast<typename Elt<DT>::H *>(lds_raw));
 }
 
 // k_rev_fstripe for batches of 64 or more: one workgroup per (phase, stripe s, slot u,
