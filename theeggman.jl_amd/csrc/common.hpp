@@ -34,8 +34,9 @@ int fail(int code, const char *fmt, ...);
 // fp-contract off, so no fused multiply-add is formed and a state's sum matches the reference's
 // _dp_states! (src/dp.jl:344-363) bit for bit when accumulated in plan order.  Complex multiply
 // is (ac - bd, ad + bc), Julia's `*` for Complex.  The product is rounded before it meets the
-// accumulator, so muladd_hh(acc, a, b) == add_h(acc, mul_hh(a, b)) bit for bit: products may be
-// formed in any order as long as they are added in the list's.
+// accumulator, so muladd_hh(acc, a, b) == add_h(acc, mul_hh(a, b)) and muladd(acc, p, h) ==
+// add_h(acc, mul_ph(p, h)) bit for bit: products may be formed in any order as long as they are
+// added in the list's.
 
 // The tail of a gather loop taken in batches of U: `batch(n)` with n an integral_constant equal to
 // rem (0 < rem < U), so the last rem loads are issued together like a full batch's instead of one
@@ -58,6 +59,7 @@ template <> struct Elt<EGGMAN_F64> {
     __device__ static inline H muladd(H acc, P p, H h) { return acc + p * h; }
     __device__ static inline H muladd_hh(H acc, H a, H b) { return acc + a * b; }
     __device__ static inline H mul_hh(H a, H b) { return a * b; }
+    __device__ static inline H mul_ph(P p, H h) { return p * h; }
     __device__ static inline H add_h(H a, H b) { return a + b; }
 };
 template <> struct Elt<EGGMAN_F32> {
@@ -68,6 +70,7 @@ template <> struct Elt<EGGMAN_F32> {
     __device__ static inline H muladd(H acc, P p, H h) { return acc + p * h; }
     __device__ static inline H muladd_hh(H acc, H a, H b) { return acc + a * b; }
     __device__ static inline H mul_hh(H a, H b) { return a * b; }
+    __device__ static inline H mul_ph(P p, H h) { return p * h; }
     __device__ static inline H add_h(H a, H b) { return a + b; }
 };
 template <> struct Elt<EGGMAN_C64> {
@@ -84,6 +87,7 @@ template <> struct Elt<EGGMAN_C64> {
     __device__ static inline H mul_hh(H a, H b) {
         return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
     }
+    __device__ static inline H mul_ph(P p, H h) { return mul_hh(p, h); }
     __device__ static inline H add_h(H a, H b) { return make_double2(a.x + b.x, a.y + b.y); }
 };
 template <> struct Elt<EGGMAN_C32> {
@@ -100,6 +104,7 @@ template <> struct Elt<EGGMAN_C32> {
     __device__ static inline H mul_hh(H a, H b) {
         return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
     }
+    __device__ static inline H mul_ph(P p, H h) { return mul_hh(p, h); }
     __device__ static inline H add_h(H a, H b) { return make_float2(a.x + b.x, a.y + b.y); }
 };
 // Exact integer walk: states are 128-bit integers modulo 2^128, pair values are int64.  The
@@ -127,6 +132,9 @@ template <> struct Elt<EGGMAN_I128> {
     }
     __device__ static inline H muladd_hh(H acc, H a, H b) { return add(acc, mul(a, b)); }
     __device__ static inline H mul_hh(H a, H b) { return mul(a, b); }
+    __device__ static inline H mul_ph(P p, H h) {
+        return mul(make_ulonglong2((unsigned long long)p, p < 0 ? ~0ull : 0ull), h);
+    }
     __device__ static inline H add_h(H a, H b) { return add(a, b); }
 };
 

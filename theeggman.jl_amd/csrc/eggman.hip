@@ -1940,6 +1940,12 @@ static void launch_rev_g(const DevCaps &caps, const Tune &tune, const DevPlan &p
 // wave-stripes hold at most four blocks: there, and for smaller batches, the phase loop would be
 // bound by latency, so the adjoint gather over the sorted lists and k_rev_fstripe run every phase
 // at once (the same sums in the same order).
+// Children per level up to which the adjoint gather of batches of at most four takes one wave per
+// child (rev_g_lanes_body): at N=28 the levels of 1, 105, 1820 and 12376 children, whose lists
+// average 105, 52, 34 and 25 entries; the larger levels' lists average 19 or fewer.  (One N=28
+// gradient: 0.277 -> 0.273 ms; at 32 instances it was 4% slower than a lane per child.)
+constexpr int64_t kGLanesMax = 16384;
+
 template <int DT, typename W>
 static void launch_rev_fused(DevState &ds, const DevPlan &pl, const Tune &tune, int L, const void *P,
                              int64_t ldp, const void *Hc, int64_t ldh, const void *Gs, void *Gc,
@@ -1978,7 +1984,11 @@ static void launch_rev_fused(DevState &ds, const DevPlan &pl, const Tune &tune, 
         // batches of at most 32: the adjoint gather and the stripe sums in one launch
         const int64_t nch = pl.lay.nst[L - 2];
         const int64_t ntiles = (nb + bt - 1) / bt;
-        const int64_t nga = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (nch * ntiles * bt + kBlock - 1) / kBlock));
+        // levels of at most kGLanesMax children (their parent lists are the longest) gather with
+        // one wave per child when the batch is at most four instances
+        const int glanes = nch <= kGLanesMax && bt <= 4 ? 1 : 0;
+        const int64_t nga = glanes ? std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (nch + kBlock / 64 - 1) / (kBlock / 64)))
+                                   : std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (nch * ntiles * bt + kBlock - 1) / kBlock));
         const int64_t ngb = std::max<int64_t>(1, std::min<int64_t>(kMaxBlocks, (int64_t)nph * f.S * f.nslot));
         const int32_t *cent = (const int32_t *)pl.d_fent;
         const int64_t *coff = (const int64_t *)pl.d_coff + (pl.lay.levels[L - 2] - 1);
@@ -1986,7 +1996,7 @@ static void launch_rev_fused(DevState &ds, const DevPlan &pl, const Tune &tune, 
     k_rev_fsmall<DT, BTV><<<dim3((unsigned)(nga + ngb)), dim3(kBlock), fstripe_lds_bytes<DT, BTV>(), st>>>( \
         nga, cent, coff, nch, (const PT *)P, ldp, (HT *)Gc, ntiles, (const int2 *)pl.d_fterms,    \
         (const int64_t *)pl.d_ftoff + f.toff, (const int64_t *)pl.d_fprow0 + f.prow, f.S, f.nslot, \
-        pl.lay.K, f.plo, nph, (const HT *)Hc, ldh, (const HT *)Gs, ldg, (HT *)part, nb)
+        pl.lay.K, f.plo, nph, (const HT *)Hc, ldh, (const HT *)Gs, ldg, (HT *)part, nb, glanes)
         switch (bt) {
         case 1: EGG_FSMALL(1); break;
         case 2: EGG_FSMALL(2); break;

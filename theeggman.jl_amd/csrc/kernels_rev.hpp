@@ -874,7 +874,50 @@ __device__ inline void fstripe_lds_body(
     }
 }
 
-// LDS bytes of fstripe_lds_body<DT, BT>.
+// The adjoint gather for batches of at most 32 on levels of few children with long parent lists
+// (k_rev_fsmall): one wave per child, a chunk of CT list entries multiplied out with the lanes over
+// (entry, instance) pairs into LDS, then each instance's lane adds the chunk's products in list
+// order.  The sums of rev_g_bt_body to the bit, with one round trip per chunk instead of one per
+// eight entries (lists of up to 105 entries at N=28).
+template <int DT, int BT>
+__device__ inline void rev_g_lanes_body(
+    int64_t bid, int64_t nbid, const int32_t *__restrict__ cent, const int64_t *__restrict__ coff,
+    int64_t nch, const typename Elt<DT>::P *__restrict__ P, int64_t ldp,
+    const typename Elt<DT>::H *__restrict__ Gs, int64_t ldg, typename Elt<DT>::H *__restrict__ Gc,
+    int64_t nb, typename Elt<DT>::H *lds) {
+    using E = Elt<DT>;
+    using HT = typename E::H;
+    constexpr int bits = Packing<int32_t>::bits;
+    constexpr int32_t mask = (1 << bits) - 1;
+    constexpr int CT = BT <= 16 ? 64 : 32;
+    constexpr int R = (CT * BT + 63) / 64;
+    constexpr int WPB = kBlock / 64;
+    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    HT *buf = lds + (size_t)w * CT * BT;
+    for (int64_t c = bid * WPB + w; c < nch; c += nbid * WPB) {
+        const int64_t e0 = coff[c], e1 = coff[c + 1];
+        HT acc = E::zero();
+        for (int64_t c0 = e0; c0 < e1; c0 += CT) {
+            const int n = (int)min((int64_t)CT, e1 - c0);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int idx = lane + 64 * r, t = idx / BT, b = idx % BT;
+                if (t < n && b < nb) {
+                    const int32_t e = cent[c0 + t];
+                    buf[t * BT + b] = E::mul_ph(P[((int64_t)(e & mask) - 1) * ldp + b],
+                                                Gs[(int64_t)(e >> bits) * ldg + b]);
+                }
+            }
+            wave_lds_order();
+            if (lane < nb)
+                for (int t = 0; t < n; t++) acc = E::add_h(acc, buf[t * BT + lane]);
+            wave_lds_order();
+        }
+        if (lane < nb) Gc[c * ldg + lane] = acc;
+    }
+}
+
+// LDS bytes of fstripe_lds_body<DT, BT> (rev_g_lanes_body uses a part of them).
 template <int DT, int BT>
 constexpr size_t fstripe_lds_bytes() {
     return (size_t)kFuseWaves * ((BT <= 16 ? 64 : 32) + 1) * BT * sizeof(typename Elt<DT>::H);
@@ -891,7 +934,8 @@ __global__ __launch_bounds__(kBlock) void k_rev_fstripe(
 }
 
 // The small-batch reverse step of one level in one launch: workgroups [0, nga) run the adjoint
-// gather over the sorted lists (BT lanes per child), the others the stripe sums (one workgroup per
+// gather over the sorted lists (BT lanes per child, or with `glanes` one wave per child,
+// rev_g_lanes_body), the others the stripe sums (one workgroup per
 // (phase, stripe, slot), fstripe_lds_body, with fstripe_lds_bytes<DT, BT>() of dynamic LDS); both
 // read only G of the parents' level, so they are independent.
 template <int DT, int BT>
@@ -902,11 +946,15 @@ __global__ __launch_bounds__(kBlock) void k_rev_fsmall(
     const int64_t *__restrict__ prow0, int S, int nslot, int K, int plo, int nph,
     const typename Elt<DT>::H *__restrict__ Hc, int64_t ldh,
     const typename Elt<DT>::H *__restrict__ Gs, int64_t ldg, typename Elt<DT>::H *__restrict__ part,
-    int64_t nb) {
+    int64_t nb, int glanes) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    if ((int64_t)blockIdx.x < nga)
-        rev_g_bt_body<DT, int32_t, BT>(blockIdx.x, nga, cent, coff, nch, P, ldp, Gs, ldg, Gc, nb, ntiles);
-    else
+    if ((int64_t)blockIdx.x < nga) {
+        if (glanes)
+            rev_g_lanes_body<DT, BT>(blockIdx.x, nga, cent, coff, nch, P, ldp, Gs, ldg, Gc, nb,
+                                     reinterpret_cast<typename Elt<DT>::H *>(lds_raw));
+        else
+            rev_g_bt_body<DT, int32_t, BT>(blockIdx.x, nga, cent, coff, nch, P, ldp, Gs, ldg, Gc, nb, ntiles);
+    } else
         fstripe_lds_body<DT, BT>(blockIdx.x - nga, (int64_t)gridDim.x - nga, terms, toff, prow0, S,
                                  nslot, K, plo, nph, Hc, ldh, Gs, ldg, part, nb,
                                  reinterpret_cast<typename Elt<DT>::H *>(lds_raw));
