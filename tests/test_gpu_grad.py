@@ -74,7 +74,8 @@ def test_fused_gradient_is_exact_on_integer_matrices(T, N, B, sparsity):
     # component complex: about 10^5 and 10^3), so the
     # floating-point gradient of the fused reverse step (degrees 26 and up) must equal the exact
     # one bit for bit whatever its summation order -- through the fused kernel (passes of 64 and
-    # more) and through the small-batch kernels (passes of 1 and 7)
+    # more) and through the small-batch kernels (passes of 7, and of 1 and 3, whose small levels
+    # gather with one wave per child)
     note(f"fused grad exact {np.dtype(T)} N={N} B={B}")
     if np.dtype(T).kind == "c":
         As = np.stack([sparse_adjacency(N, 71, b, sparsity) + 1j * sparse_adjacency(N, 72, b, sparsity)
@@ -84,7 +85,7 @@ def test_fused_gradient_is_exact_on_integer_matrices(T, N, B, sparsity):
     P = te._gather_pairs_batch(T, As, None, N)
     plan = te._dp_plan(N)
     runs = [te._haf_dp_backend_grad(BE, P, plan)]
-    for mb in ((64, 7) if B <= 100 else (512, 64)):
+    for mb in ((64, 7, 3, 1) if B <= 100 else (512, 64)):
         runs.append(te._haf_dp_backend_grad(BE, P, plan, max_batch=mb))
     if np.dtype(T).kind == "c":
         # the deletion hafnians of a few instances from the oracle's complex128 walk (exact here)
