@@ -65,7 +65,8 @@ def sparse_adjacency(N, config, b, sparsity):
 
 
 @pytest.mark.parametrize("T,N,B,sparsity", [(np.float64, 26, 70, 1), (np.float64, 28, 1024, 1),
-                                            (np.float32, 26, 70, 2), (np.complex64, 26, 70, 4)])
+                                            (np.float32, 26, 70, 2), (np.complex64, 26, 70, 4),
+                                            (np.float32, 26, 300, 2), (np.complex64, 26, 300, 4)])
 def test_fused_gradient_is_exact_on_integer_matrices(T, N, B, sparsity):
     # 0/1 adjacency matrices (Gaussian integers a + bi, a, b in {0, 1}, for complex): every value
     # and partial sum of the walk and of the reverse sweep is a non-negative integer (a Gaussian
@@ -73,9 +74,10 @@ def test_fused_gradient_is_exact_on_integer_matrices(T, N, B, sparsity):
     # for the sparser graphs of the single-precision cases (G(26, 1/4) real, G(26, 1/16) per
     # component complex: about 10^5 and 10^3), so the
     # floating-point gradient of the fused reverse step (degrees 26 and up) must equal the exact
-    # one bit for bit whatever its summation order -- through the fused kernel (passes of 64 and
-    # more) and through the small-batch kernels (passes of 7, and of 1 and 3, whose small levels
-    # gather with one wave per child)
+    # one bit for bit whatever its summation order -- through the fused kernel (passes of four
+    # tiles of 64 and more), through the stripe lists (passes of 64 to 255) and through the
+    # small-batch kernels (passes of 7, and of 1 and 3, whose small levels gather with one wave per
+    # child)
     note(f"fused grad exact {np.dtype(T)} N={N} B={B}")
     if np.dtype(T).kind == "c":
         As = np.stack([sparse_adjacency(N, 71, b, sparsity) + 1j * sparse_adjacency(N, 72, b, sparsity)
@@ -85,7 +87,7 @@ def test_fused_gradient_is_exact_on_integer_matrices(T, N, B, sparsity):
     P = te._gather_pairs_batch(T, As, None, N)
     plan = te._dp_plan(N)
     runs = [te._haf_dp_backend_grad(BE, P, plan)]
-    for mb in ((64, 7, 3, 1) if B <= 100 else (512, 64)):
+    for mb in ((64, 7, 3, 1) if B <= 100 else (256, 64) if B < 1024 else (512, 64)):
         runs.append(te._haf_dp_backend_grad(BE, P, plan, max_batch=mb))
     if np.dtype(T).kind == "c":
         # the deletion hafnians of a few instances from the oracle's complex128 walk (exact here)
@@ -180,6 +182,23 @@ def test_gradient_is_deterministic_and_chunk_independent(N):
         v, g = te._haf_dp_backend_grad(BE, P, te._dp_plan(N), max_batch=mb)
         assert v.tobytes() == v1.tobytes(), mb
         assert np.asarray(g).tobytes() == np.asarray(g1).tobytes(), mb
+
+
+@pytest.mark.parametrize("N,T", [(26, np.float64), (28, np.float64), (28, np.float32)])
+def test_fused_kernel_and_stripe_lists_agree(N, T):
+    # a pass of five tiles takes the fused kernel, passes of 64 the stripe lists, passes of 256
+    # (four tiles) and of 44 one each, passes of 7 the small-batch kernels: the same sums in the
+    # same order, so the same bits on random matrices
+    B = 300
+    note(f"fused vs lists N={N} {np.dtype(T)}")
+    As = np.stack([synth.sym_uniform(N, 33, b) for b in range(B)]).astype(T)
+    P = te._gather_pairs_batch(T, As, None, N)
+    plan = te._dp_plan(N)
+    v0, g0 = te._haf_dp_backend_grad(BE, P, plan)
+    for mb in (256, 64, 7):
+        v, g = te._haf_dp_backend_grad(BE, P, plan, max_batch=mb)
+        assert np.asarray(v).tobytes() == np.asarray(v0).tobytes(), mb
+        assert np.asarray(g).tobytes() == np.asarray(g0).tobytes(), mb
 
 
 def test_repeated_gradient_folds_the_expanded_gradient():

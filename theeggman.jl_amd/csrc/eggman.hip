@@ -1936,10 +1936,16 @@ static void launch_rev_g(const DevCaps &caps, const Tune &tune, const DevPlan &p
 }
 
 // The reverse step of level L by the fused order: G of level L-1 into Gc and the level's dP
-// partial rows.  Batches of 64 instances or more take k_rev_fused, except on levels whose
-// wave-stripes hold at most four blocks: there, and for smaller batches, the phase loop would be
-// bound by latency, so the adjoint gather over the sorted lists and k_rev_fstripe run every phase
-// at once (the same sums in the same order).
+// partial rows.  Passes of kFusedMinTiles tiles of 64 instances or more take k_rev_fused, except on
+// levels whose wave-stripes hold at most four blocks: there, and for smaller passes, the phase
+// loop would be bound by latency (a tile's stripes are S workgroups, too few to fill the device
+// with one or two tiles), so the adjoint gather over the sorted lists and the stripe sums run
+// every phase at once (the same sums in the same order).
+//
+// Forward + gradient, ms, fused kernel from one tile / from four: N=28 B=64 2.79 / 1.57, B=128
+// 3.08 / 2.78; N=32 B=64 18.7 / 12.3; at four tiles and more the fused kernel wins (N=28 B=256 3.62
+// against 5.55 with the lists, N=32 B=256 25.2 against 47.5, N=26 B=256 1.61 against 1.90).
+constexpr int64_t kFusedMinTiles = 4;
 // Children per level up to which the adjoint gather of batches of at most four takes one wave per
 // child (rev_g_lanes_body): at N=28 the levels of 1, 105, 1820 and 12376 children, whose lists
 // average 105, 52, 34 and 25 entries; the larger levels' lists average 19 or fewer.  (One N=28
@@ -1956,7 +1962,7 @@ static void launch_rev_fused(DevState &ds, const DevPlan &pl, const Tune &tune, 
     const FuseLevel &f = pl.flv[L];
     const int nph = f.phi - f.plo + 1;
     const int64_t V = (int64_t)f.S * kFuseWaves;
-    if (nb >= 64 && f.nblk > 4 * V) {
+    if (nb >= 64 * kFusedMinTiles && f.nblk > 4 * V) {
         const int64_t ntiles = (nb + 63) / 64;
         const size_t lds = (size_t)f.nslot * 64 * (sizeof(PT) + kFuseWaves * sizeof(HT));
         const int64_t grid = ntiles >= 8 ? 8 * (int64_t)f.S * ((ntiles + 7) / 8)
