@@ -1936,16 +1936,17 @@ static void launch_rev_g(const DevCaps &caps, const Tune &tune, const DevPlan &p
 }
 
 // The reverse step of level L by the fused order: G of level L-1 into Gc and the level's dP
-// partial rows.  Passes of kFusedMinTiles tiles of 64 instances or more take k_rev_fused, except on
-// levels whose wave-stripes hold at most four blocks: there, and for smaller passes, the phase
-// loop would be bound by latency (a tile's stripes are S workgroups, too few to fill the device
-// with one or two tiles), so the adjoint gather over the sorted lists and the stripe sums run
-// every phase at once (the same sums in the same order).
+// partial rows.  Passes of fused_min_tiles(K) tiles of 64 instances or more take k_rev_fused,
+// except on levels whose wave-stripes hold at most four blocks: there, and for smaller passes, the
+// phase loop would be bound by latency (a tile's stripes are S workgroups, too few to fill the
+// device with one or two tiles), so the adjoint gather over the sorted lists and the stripe sums
+// run every phase at once (the same sums in the same order).
 //
-// Forward + gradient, ms, fused kernel from one tile / from four: N=28 B=64 2.79 / 1.57, B=128
-// 3.08 / 2.78; N=32 B=64 18.7 / 12.3; at four tiles and more the fused kernel wins (N=28 B=256 3.62
-// against 5.55 with the lists, N=32 B=256 25.2 against 47.5, N=26 B=256 1.61 against 1.90).
-constexpr int64_t kFusedMinTiles = 4;
+// Forward + gradient, ms, fused kernel against the lists: one tile, N=28 B=64 2.79 / 1.57, N=32
+// B=64 18.7 / 12.3; two tiles, N=26 B=128 1.31 / 1.13, N=28 B=128 3.08 / 2.78, but N=32 B=128
+// 20.8 / 23.9; three tiles, N=26 B=192 1.51 / 1.82, N=28 B=192 3.41 / 5.26, N=32 B=192 23.3 /
+// 41.6; four and more, N=28 B=256 3.62 / 5.55, N=32 B=256 25.2 / 47.5.
+static int64_t fused_min_tiles(int K) { return K >= 30 ? 2 : 3; }
 // Children per level up to which the adjoint gather of batches of at most four takes one wave per
 // child (rev_g_lanes_body): at N=28 the levels of 1, 105, 1820 and 12376 children, whose lists
 // average 105, 52, 34 and 25 entries; the larger levels' lists average 19 or fewer.  (One N=28
@@ -1962,7 +1963,7 @@ static void launch_rev_fused(DevState &ds, const DevPlan &pl, const Tune &tune, 
     const FuseLevel &f = pl.flv[L];
     const int nph = f.phi - f.plo + 1;
     const int64_t V = (int64_t)f.S * kFuseWaves;
-    if (nb >= 64 * kFusedMinTiles && f.nblk > 4 * V) {
+    if (nb >= 64 && (nb + 63) / 64 >= fused_min_tiles(pl.lay.K) && f.nblk > 4 * V) {
         const int64_t ntiles = (nb + 63) / 64;
         const size_t lds = (size_t)f.nslot * 64 * (sizeof(PT) + kFuseWaves * sizeof(HT));
         const int64_t grid = ntiles >= 8 ? 8 * (int64_t)f.S * ((ntiles + 7) / 8)
@@ -2115,9 +2116,13 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
             const int ns = lb <= lt && np == 1 ? pass_streams(DT, (nb + sub - 1) / sub) : 1;
             const hipStream_t st0 = st;
             EGG_TRY(fork_side(ds, st0, ns));
-            for (int64_t s0 = 0, i = 0; s0 < nb && lb <= lt; s0 += sub, i++) {
+            for (int64_t s0 = 0, i = 0, n; s0 < nb && lb <= lt; s0 += n, i++) {
+                // a remainder of fewer than 64 instances joins the sub-pass before it (alone it
+                // would take the small-batch kernels, one lane per instance)
+                n = std::min(sub, nb - s0);
+                if (nb - s0 - n > 0 && nb - s0 - n < 64) n = nb - s0;
                 st = split_stream(ds, st0, (int)(i % ns));
-                for (int L = lb; L <= lt; L++) launch_cols(L, s0, std::min(sub, nb - s0));
+                for (int L = lb; L <= lt; L++) launch_cols(L, s0, n);
             }
             st = st0;
             EGG_TRY(join_side(ds, st, ns));
