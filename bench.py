@@ -444,12 +444,52 @@ def run_plans(path):
     print(json.dumps(res, default=str), flush=True)
 
 
+def run_sweep(path, K=28):
+    """Forward and forward + full gradient of real FP64 degree-K batches over a range of batch
+    sizes (device times, inputs resident in HBM, tuned launches), written to `path` as JSON."""
+    import numpy as np
+    import theeggman as te
+    from theeggman import _lib, hip, synth
+    st = hip.Stream()
+    be = te.HIPBackend()
+    plan = te._dp_plan(K)
+    h = te._device_plan(be, plan)
+    _, nt = te._dp_counts(K)
+    npairs = K * (K - 1) // 2
+    rows = []
+    for B in (1, 2, 4, 8, 16, 32, 64, 128, 192, 256, 384, 512, 1024, 2048, 4096):
+        As = np.stack([synth.sym_normal(K, 9, b, 1 / np.sqrt(K)) for b in range(B)])
+        P = synth.pair_batch(As).astype(np.float64)
+        Pd = hip.DeviceBuffer.from_array(P)
+        out = hip.DeviceBuffer(B * 8)
+        dP = hip.DeviceBuffer(B * 8 * npairs)
+        steps = 50 if B <= 64 else 10 if B <= 1024 else 4
+        fms, _ = time_device(hip, st, lambda: te._haf_dp_device(be, plan, np.float64, Pd.addr, B, B,
+                                                                out.addr, st.handle), steps, 2)
+        gms, _ = time_device(hip, st, lambda: _lib.check(_lib.lib().eggman_dp_forward_grad_device(
+            h, _lib.F64, Pd.addr, B, B, out.addr, dP.addr, B, st.handle), "grad"), max(2, steps // 2), 1)
+        row = {"B": B, "forward_ms": fms, "hafnians_per_s": B / (fms / 1e3),
+               "transitions_per_s": B * nt / (fms / 1e3), "forward_grad_ms": gms,
+               "hafnians_with_gradient_per_s": B / (gms / 1e3)}
+        rows.append(row)
+        log(f"sweep K={K} B={B}: forward {fms:.3f} ms, forward+gradient {gms:.3f} ms")
+        del Pd, out, dP
+    res = {"workload": f"real FP64 N={K} batches, N(0,1/N), tuned launches, inputs resident in HBM",
+           "rows": rows}
+    with open(path, "w") as f:
+        json.dump(res, f, indent=1)
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--configs", metavar="PATH",
                     help="instead of the benchmark line, measure the five workload configurations "
                          "and write them to PATH")
+    ap.add_argument("--sweep", metavar="PATH",
+                    help="instead of the benchmark line, time the forward and the forward + "
+                         "gradient of N=28 FP64 batches of 1 to 4096 and write them to PATH")
     ap.add_argument("--plans", metavar="PATH",
                     help="instead of the benchmark line, measure plan build/upload/generation "
                          "times, upload amortisation and device memory, and write them to PATH")
@@ -475,6 +515,9 @@ def main():
         sys.exit(2)
     if args.configs:
         run_configs(args.configs)
+        return
+    if args.sweep:
+        run_sweep(args.sweep)
         return
     if args.plans:
         run_plans(args.plans)
