@@ -232,10 +232,12 @@ def test_device_resident_entry_point():
 
 @pytest.mark.parametrize("T,N,B", [(np.float64, 26, 100), (np.float64, 18, 70), (np.float32, 30, 64),
                                    (np.complex64, 26, 80), (np.float64, 18, 1), (np.float64, 28, 8),
-                                   (np.complex128, 24, 33)])
+                                   (np.complex128, 24, 33), (np.float64, 28, 24), (np.float32, 26, 17),
+                                   (np.complex64, 30, 40)])
 def test_untuned_defaults_bit_exact(T, N, B):
     # degrees and element types with no tuning entry use the default launch choices (the fused
-    # walk, or the 16-byte-load kernel in passes); the values are the oracle's bits
+    # walk, the 16-byte-load kernel on 16-instance tiles for 16 to 63 instances of degree 26 and
+    # up, or that kernel in passes); the values are the oracle's bits
     note(f"defaults {np.dtype(T)} N={N} B={B}")
     assert te.tune_get(N, T, B) is None
     As = stack(N, B, 15, "complex" if np.dtype(T).kind == "c" else "normal").astype(T)

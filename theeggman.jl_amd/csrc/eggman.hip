@@ -638,6 +638,18 @@ static Tune default_tune(const PlanLayout &lay, int, int64_t B) {
     t.chunk = B;
     t.fused = true;
     t.lds = B >= 64;
+    if (B >= 16 && B < 64 && lay.K >= 26) {
+        // 16 to 63 instances of a degree whose levels are large: the 16-byte-load kernel on
+        // 16-instance tiles (one to four of them) rather than a lane per instance.  N=28: 16
+        // instances 0.152 -> 0.121 ms, 32 0.250 -> 0.157; N=32: 16 0.90 -> 0.47, 24 1.69 -> 0.90
+        // (at N=24 both take about the same time, and small degrees take the fused kernel)
+        t.bt = 64;
+        t.lds = true;
+        t.vec = true;
+        t.deep = true;
+        t.occ = 2;
+        return t;
+    }
     if (B >= 64) {
         t.lds = true;
         t.vec = true;
