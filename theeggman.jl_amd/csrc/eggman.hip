@@ -2114,8 +2114,10 @@ static int forward_grad_levels(DevState &ds, DevPlan &pl, const void *P, int64_t
             const char *Ps = Pc + (size_t)s0 * pb;
             const void *Hc = level_rows(L - 1) + s0;
             void *Ho = L == top ? (void *)(outc + s0) : (void *)(level_rows(L) + s0);
-            // the LDS kernels work on tiles of 16 or 64 instances: fewer take the small-batch ones
-            if (!(tune.lds && ns >= 64 &&
+            // the LDS kernels work on tiles of 64 instances (16 for the 16-byte-load kernel):
+            // fewer take the small-batch ones (N=28, 16 to 48 instances, forward + gradient with
+            // the 16-byte-load kernel from 16: 4-8% faster)
+            if (!(tune.lds && (ns >= 64 || (tune.vec && ns >= 16)) &&
                   launch_level_lds<DT, W, 1>(ds.caps, tune, pl, L, Ps, ldp, Hc, ldh, Ho, ldh, ns, st)))
                 launch_level_bt<DT, W>(pow2_at_least(ns), pl, L, Ps, ldp, Hc, ldh, Ho, ldh, ns, st);
         };
