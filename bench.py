@@ -344,6 +344,7 @@ def run_configs(path):
     ms, B = dev_forward(As, 40, np.int64, 2, backend=gen)
     res["5b_exact_n40_b64"] = {"gpu_ms": ms, "hafnians_per_s": B / ms * 1e3,
                                "transitions_per_s": B * te._dp_counts(40)[1] / ms * 1e3}
+    res["command"] = "python bench.py --configs PATH"
     with open(path, "w") as f:
         json.dump(res, f, indent=1)
     print(json.dumps(res), flush=True)
@@ -439,6 +440,7 @@ def run_plans(path):
                     "reference holds every state of every instance (dp_batch_bytes)"}
         log(f"memory K={K}: {res['memory'][K]}")
         del Pd, out, dP
+    res["command"] = "python bench.py --plans PATH"
     with open(path, "w") as f:
         json.dump(res, f, indent=1, default=str)
     print(json.dumps(res, default=str), flush=True)
@@ -475,7 +477,7 @@ def run_sweep(path, K=28):
         log(f"sweep K={K} B={B}: forward {fms:.3f} ms, forward+gradient {gms:.3f} ms")
         del Pd, out, dP
     res = {"workload": f"real FP64 N={K} batches, N(0,1/N), tuned launches, inputs resident in HBM",
-           "rows": rows}
+           "command": "python bench.py --sweep PATH", "rows": rows}
     with open(path, "w") as f:
         json.dump(res, f, indent=1)
     print(json.dumps(res), flush=True)
