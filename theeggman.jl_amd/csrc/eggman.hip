@@ -1015,8 +1015,10 @@ static bool launch_fused(const DevCaps &caps, const DevPlan &pl, const void *P, 
 // bottom and at the top of the plan with at most kFuseLevel states each walk in one launch of
 // k_fused_range per run, their values in at most 64 KB of LDS, instead of one launch per level.
 // One N=28 FP64 matrix: 67.2 -> 64.2 us; N=24: 46.3 -> 40.8 us (512 or 4096 states per level
-// fused less or made the single workgroup the bottleneck).
-constexpr int64_t kFuseLevel = 1024;
+// fused less or made the single workgroup the bottleneck).  2048 rather than 1024 also takes N=28's
+// second bottom level (1820 states): 52.6 -> 50.4 us (N=24 35.4 -> 34.7, N=32 unchanged; 3000:
+// 55.7 us at N=28).
+constexpr int64_t kFuseLevel = 2048;
 constexpr size_t kFuseLds = 65536;
 
 // LDS bytes of a fused run La .. Lb: levels La .. Lb-1 (and the empty set when La = 2), then P.
