@@ -289,7 +289,7 @@ static hipStream_t split_stream(const DevState &ds, hipStream_t st, int i) {
 constexpr int kPassStreams = 2;
 // Gradient passes in flight at once (forward_grad_levels): N=28 FP64, forward + gradient per
 // call, against one pass at a time with its forward sub-passes split over two streams:
-// B=1024 11.79 -> 11.65 ms (11.82 -> 11.49 in another session), B=2048 23.5 -> 22.9,
+// B=1024 11.79 -> 11.65 ms (11.82 -> 11.49 in another run), B=2048 23.5 -> 22.9,
 // N=26 B=2048 9.99 -> 9.80; splitting each pass's sub-passes too was no better (11.79, 23.3,
 // 10.07).
 constexpr int kGradStreams = 2;
