@@ -39,6 +39,13 @@ def main():
         for K in (36, 40):
             shapes.append((K, np.complex128, 1))
         shapes.append((40, np.int64, 64))
+        # degrees between the benchmark ones, and larger complex and FP32 batches
+        for B in (64, 256, 1024, 4096):
+            shapes.append((26, np.float64, B))
+        for B in (64, 256, 1024):
+            shapes.append((30, np.float64, B))
+        shapes += [(28, np.complex128, 1024), (28, np.complex64, 1024), (24, np.float32, 4096),
+                   (28, np.float32, 4096)]
     seen = set()
     for K, T, B in shapes:
         if (K, np.dtype(T).str, B) in seen:
