@@ -271,3 +271,47 @@ def test_passes_in_flight_bit_identical(T):
     assert split.tobytes() == whole.tobytes()
     if T in DT:
         assert split.tobytes() == oracle_vals(As, T).tobytes()
+
+
+def test_calls_from_several_threads():
+    # host threads calling at once, host-pointer calls and device-pointer calls on streams of
+    # their own (passes two at a time inside, the library's buffers shared): every result is the
+    # bits of the same call made alone
+    import threading
+    from theeggman import hip
+    K = 28
+    plan = te._dp_plan(K)
+    note(f"threads K={K}")
+    batches = [stack(K, 300 + 37 * i, 40 + i) for i in range(6)]
+    want = [te.hafnian(As, backend=BE, check_symmetric=False) for As in batches]
+    got, errors = [None] * 12, []
+
+    def host_call(i):
+        try:
+            got[i] = te.hafnian(batches[i % 6], backend=BE, check_symmetric=False)
+        except Exception as e:          # reported below
+            errors.append(repr(e))
+
+    def device_call(i):
+        try:
+            As = batches[i % 6]
+            P = synth.pair_batch(As)
+            Pd = hip.DeviceBuffer.from_array(P)
+            out = hip.DeviceBuffer(As.shape[0] * 8)
+            st = hip.Stream()
+            for _ in range(3):
+                te._haf_dp_device(BE, plan, np.float64, Pd.addr, As.shape[0], As.shape[0], out.addr,
+                                  st.handle)
+            st.synchronize()
+            got[i] = out.download(np.float64, As.shape[0])
+        except Exception as e:
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=host_call if i < 6 else device_call, args=(i,)) for i in range(12)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not errors, errors
+    for i in range(12):
+        assert got[i] is not None and got[i].tobytes() == want[i % 6].tobytes(), i
