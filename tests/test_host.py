@@ -296,3 +296,30 @@ def test_fused_reverse_tables_follow_the_plan(K):
     _lib.check(_lib.lib().eggman_fuse_validate(K, ctypes.byref(ne), ctypes.byref(nt)), "fuse_validate")
     ntrans = te._dp_counts(K)[1]
     assert ne.value == ntrans and nt.value == ntrans
+
+
+def test_symmetry_check_semantics():
+    # isapprox's default tolerance on the strict triangle (src/hafnian.jl:765-783); the diagonal is
+    # ignored; a batch reports the first matrix that fails, and an exactly symmetric stack passes
+    from theeggman.hafnian import _check_symmetric
+    A = synth.sym_uniform(8, 3, 0)
+    _check_symmetric(A)
+    B = A.copy()
+    B[2, 5] *= 1 + 1e-17                      # within eps
+    B[4, 4] = np.nan                          # diagonal: ignored
+    _check_symmetric(B)
+    C = A.copy()
+    C[5, 2] += 1e-6
+    with pytest.raises(te.ArgumentError, match=r"not symmetric at \(3, 6\)"):
+        _check_symmetric(C)
+    stack = np.stack([A, A, C])
+    with pytest.raises(te.ArgumentError, match=r"not symmetric at \(3, 6\)"):
+        te.hafnian(stack)
+    with pytest.raises(te.ArgumentError, match=r"not symmetric at \(3, 6\)"):
+        te.hafnian([A, C])
+    Z = (A + 1j * A).astype(np.complex128)
+    _check_symmetric(Z)
+    Z2 = Z.copy()
+    Z2[0, 7] += 1j * 1e-3
+    with pytest.raises(te.ArgumentError):
+        _check_symmetric(Z2)
