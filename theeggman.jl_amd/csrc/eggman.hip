@@ -416,6 +416,22 @@ static int run_pipe(DevState &ds, int64_t nchunks, size_t in_max, size_t out_max
         (void)hipStreamSynchronize(ds.down);
         return rc;
     };
+    if (nchunks == 1) {
+        // One chunk: both copies and the compute go in order on the call's stream.  Nothing
+        // overlaps, so the cross-stream events of the pipeline would only add their latency.
+        size_t ib = 0, ob = 0;
+        int rc = fill(0, ds.pin_in[0].p, &ib, &ob);
+        if (rc != EGGMAN_OK) return rc;
+        hipError_t e = hipSuccess;
+        if (ib) e = hipMemcpyAsync(ds.din[0].p, ds.pin_in[0].p, ib, hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) return drained(fail(EGGMAN_EHIP, "upload: %s", hipGetErrorString(e)));
+        if ((rc = run(0, ds.din[0].p, ds.dout[0].p, st)) != EGGMAN_OK) return drained(rc);
+        if (ob) e = hipMemcpyAsync(ds.pin_out[0].p, ds.dout[0].p, ob, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess)
+            return drained(fail(EGGMAN_EHIP, "download: %s", hipGetErrorString(e)));
+        return drain(0, ds.pin_out[0].p);
+    }
     for (int64_t c = 0; c < nchunks; c++) {
         const int s = (int)(c & 1);
         int rc = EGGMAN_OK;

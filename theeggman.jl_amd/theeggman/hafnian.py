@@ -50,7 +50,7 @@ def _check_symmetric(A: np.ndarray) -> None:
     """Symmetric to within isapprox's default tolerance (src/hafnian.jl:765-783): exact equality
     first, then |x-y|^2 <= eps * max(|x|^2, |y|^2)."""
     n = A.shape[0]
-    if n < 2 or np.array_equal(A, A.T):
+    if n < 2 or (A == A.T).all():
         return
     iu = _triu(n)
     x = A[iu]
@@ -122,12 +122,12 @@ def _gather_pairs_batch(T, A_or_As, idxs, K: int) -> np.ndarray:
     if isinstance(A_or_As, np.ndarray) and A_or_As.ndim == 2:
         A = A_or_As
         if idxs is None:
-            cols = [A[iu, ju]]
-        else:
-            cols = []
-            for idx in idxs:
-                r = np.asarray(idx, dtype=np.int64) - 1
-                cols.append(A[r[iu], r[ju]])
+            # one row: a (1, npairs) array is already in [pair][batch] order
+            return A[iu, ju].astype(T, copy=False).reshape(1, -1)
+        cols = []
+        for idx in idxs:
+            r = np.asarray(idx, dtype=np.int64) - 1
+            cols.append(A[r[iu], r[ju]])
         Pt = np.stack(cols, axis=0)
     else:
         As = A_or_As
