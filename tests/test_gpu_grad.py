@@ -391,3 +391,22 @@ def test_fused_gradient_with_64bit_level_offsets():
     v2, g2 = te._haf_dp_backend_grad(BE, P, plan, max_batch=512)
     assert np.asarray(v1).tobytes() == np.asarray(v2).tobytes()
     assert np.asarray(g1).tobytes() == np.asarray(g2).tobytes()
+
+
+def test_grad_batch_forms_agree():
+    # a (B, N, N) stack, a list of the same matrices and one matrix at a time give the same bits;
+    # G is symmetric with a zero diagonal; a stack with one asymmetric matrix is refused
+    note("hafnian_grad batch forms: N=12 B=5")
+    As = np.stack([synth.sym_normal(12, 9, b, 0.3) for b in range(5)])
+    v1, G1 = te.hafnian_grad(As)
+    v2, G2 = te.hafnian_grad(list(As))
+    assert v1.tobytes() == v2.tobytes() and G1.tobytes() == G2.tobytes()
+    for b in range(5):
+        v, G = te.hafnian_grad(As[b])
+        assert v == v1[b] and G.tobytes() == G1[b].tobytes()
+    assert np.array_equal(G1, G1.transpose(0, 2, 1))
+    assert not np.diagonal(G1, axis1=1, axis2=2).any()
+    bad = As.copy()
+    bad[3, 0, 1] += 1.0
+    with pytest.raises(te.ArgumentError, match=r"not symmetric at \(1, 2\)"):
+        te.hafnian_grad(bad)

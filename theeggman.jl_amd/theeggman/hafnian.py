@@ -389,17 +389,27 @@ def hafnian_grad(A, *, check_symmetric: bool = True, backend=None, max_batch: in
     reverse sweep over the same plan on the device (new: the reference has no derivative).
     """
     batch = _is_batch(A)
-    As = [_as_matrix(M) for M in A] if batch else [_as_matrix(A)]
-    B = len(As)
-    if B == 0:
-        return np.empty(0), np.empty((0, 0, 0))
-    T = _haf_eltype(np.result_type(*[M.dtype for M in As]))
-    N = _checksquare(As[0])
-    for b, M in enumerate(As):
-        if M.shape != (N, N):
-            raise DimensionMismatch(f"matrix {b + 1} is {M.shape[0]}×{M.shape[-1]}, expected "
-                                    f"{N}×{N}; a batch shares one plan")
-        if check_symmetric:
+    if batch and isinstance(A, np.ndarray):
+        S = _as_matrix(A)                       # a (B, N, N) stack: one element type and shape
+        As = S
+        if S.shape[0] == 0:
+            return np.empty(0), np.empty((0, 0, 0))
+        T = _haf_eltype(S.dtype)
+        N = _checksquare(S[0])
+    else:
+        As = [_as_matrix(M) for M in A] if batch else [_as_matrix(A)]
+        if len(As) == 0:
+            return np.empty(0), np.empty((0, 0, 0))
+        T = _haf_eltype(np.result_type(*[M.dtype for M in As]))
+        N = _checksquare(As[0])
+        for b, M in enumerate(As):
+            if M.shape != (N, N):
+                raise DimensionMismatch(f"matrix {b + 1} is {M.shape[0]}×{M.shape[-1]}, "
+                                        f"expected {N}×{N}; a batch shares one plan")
+        S = np.stack(As)
+    B = S.shape[0]
+    if check_symmetric and not np.array_equal(S, S.transpose(0, 2, 1)):
+        for M in As:
             _check_symmetric(M)
     if N % 2 or N == 0:
         vals = np.full(B, 1 if N == 0 else 0, T)
@@ -407,11 +417,14 @@ def hafnian_grad(A, *, check_symmetric: bool = True, backend=None, max_batch: in
     else:
         _check_method("dp", N)
         _route("dp", N)
-        P = _gather_pairs_batch(T, np.stack(As), None, N)
+        P = _gather_pairs_batch(T, S, None, N)
         vals, dP = _haf_dp_backend_grad(_backend_or_default(backend), P, _plan_for(N, backend),
                                         max_batch=max_batch)
-        idx = np.arange(1, N + 1)
-        grads = np.stack([_scatter_pair_grad(dP[b], idx, N, T) for b in range(B)])
+        # identity gather: every pair (i < j) is its own matrix entry, written on both sides
+        iu, ju = _triu(N)
+        grads = np.zeros((B, N, N), T)
+        grads[:, iu, ju] = dP
+        grads[:, ju, iu] = dP
     return (vals, grads) if batch else (vals[0], grads[0])
 
 
