@@ -200,14 +200,16 @@ def _hafnian_batch(As, *, method, check_symmetric, backend, max_batch):
     T = _haf_eltype(np.result_type(*[A.dtype for A in As]) if isinstance(As, list)
                     else As.dtype)
     N = _checksquare(As[0])
+    if isinstance(As, list):
+        for b, A in enumerate(As):
+            if A.ndim != 2 or A.shape != (N, N):
+                raise DimensionMismatch(f"matrix {b + 1} is {A.shape[0]}×{A.shape[-1]}, "
+                                        f"expected {N}×{N}; a batch shares one strategy and one "
+                                        "plan")
+    stack = np.asarray(As) if isinstance(As, list) else As
     # a stack whose every matrix equals its transpose needs no per-matrix check
-    exact = (check_symmetric and isinstance(As, np.ndarray) and As.ndim == 3
-             and np.array_equal(As, As.transpose(0, 2, 1)))
-    for b, A in enumerate(As):
-        if A.ndim != 2 or A.shape != (N, N):
-            raise DimensionMismatch(f"matrix {b + 1} is {A.shape[0]}×{A.shape[-1]}, expected "
-                                    f"{N}×{N}; a batch shares one strategy and one plan")
-        if check_symmetric and not exact:
+    if check_symmetric and not np.array_equal(stack, stack.transpose(0, 2, 1)):
+        for A in As:
             _check_symmetric(A)
     if N == 0:
         return np.ones(B, T)
@@ -215,7 +217,6 @@ def _hafnian_batch(As, *, method, check_symmetric, backend, max_batch):
         return np.zeros(B, T)
     _check_method(method, N)
     _route(method, N)
-    stack = np.asarray(As) if isinstance(As, list) else As
     P = _gather_pairs_batch(T, stack, None, N)
     return _haf_dp_backend(_backend_or_default(backend), P, _plan_for(N, backend), max_batch=max_batch)
 
@@ -258,16 +259,24 @@ def _hafnian_repeated_batch(A, rpts, *, method, check_symmetric, backend, max_ba
         return np.empty(0, T)
     N = _checksquare(A)
     total = int(np.sum(rpts[0]))
-    for b, rpt in enumerate(rpts):
-        if len(rpt) != N:
-            raise DimensionMismatch(f"rpt {b + 1} has length {len(rpt)}, expected {N}")
-        r = np.asarray(rpt, dtype=np.int64)
-        if (r < 0).any():
-            raise ArgumentError(f"rpt {b + 1} contains a negative entry")
-        if int(r.sum()) != total:
-            raise ArgumentError(f"rpt {b + 1} sums to {int(r.sum())}, but rpt 1 sums to {total}; "
-                                "a batch shares one plan, so every pattern must have the same "
-                                "total")
+    try:
+        R = np.asarray(rpts, dtype=np.int64)
+    except (ValueError, TypeError):
+        R = None
+    if not (R is not None and R.ndim == 2 and R.shape[1] == N and not (R < 0).any()
+            and (R.sum(axis=1) == total).all()):
+        # find and name the first bad pattern
+        for b, rpt in enumerate(rpts):
+            if len(rpt) != N:
+                raise DimensionMismatch(f"rpt {b + 1} has length {len(rpt)}, expected {N}")
+            r = np.asarray(rpt, dtype=np.int64)
+            if (r < 0).any():
+                raise ArgumentError(f"rpt {b + 1} contains a negative entry")
+            if int(r.sum()) != total:
+                raise ArgumentError(f"rpt {b + 1} sums to {int(r.sum())}, but rpt 1 sums to "
+                                    f"{total}; a batch shares one plan, so every pattern must "
+                                    "have the same total")
+        R = np.stack([np.asarray(r, dtype=np.int64) for r in rpts])
     if check_symmetric:
         _check_symmetric(A)
     if total == 0:
@@ -279,7 +288,8 @@ def _hafnian_repeated_batch(A, rpts, *, method, check_symmetric, backend, max_ba
         _route(method, total)
     # the batched repeated form sends every total up to DP_HARD_MAX to the device under :auto
     # (src/hafnian.jl:1348); A and the index lists go to the device, which gathers P itself
-    idxs = np.stack([_expanded_indices(np.asarray(r, dtype=np.int64), total) for r in rpts])
+    # every row is 1..N with row i repeated R[b, i] times (_expanded_indices of each pattern)
+    idxs = np.repeat(np.tile(np.arange(1, N + 1, dtype=np.int64), B), R.ravel()).reshape(B, total)
     return _haf_dp_backend_repeated(_backend_or_default(backend), A.astype(T, copy=False), idxs,
                                     _plan_for(total, backend), max_batch=max_batch)
 
@@ -341,21 +351,26 @@ def hafnian_mixed(As, *, check_symmetric: bool = True, backend=None, max_batch: 
     T = _haf_eltype(np.result_type(*[A.dtype for A in As]))
     out = np.zeros(len(As), T)
     groups: dict = {}
-    for i, A in enumerate(As):
-        N = _checksquare(A)
-        if check_symmetric:
-            _check_symmetric(A)
+    sizes = [_checksquare(A) for A in As]
+    for i, N in enumerate(sizes):
         if N == 0:
             out[i] = 1
         elif N % 2 == 0:
             _check_method("dp", N)
             groups.setdefault(N, []).append(i)
+    stacks = {N: np.stack([As[i] for i in idx]) for N, idx in groups.items()}
+    if check_symmetric:
+        # when every degree's stack equals its transpose, only the odd-sized matrices are left
+        exact = all(np.array_equal(S, S.transpose(0, 2, 1)) for S in stacks.values())
+        for A, N in zip(As, sizes):
+            if not exact or N % 2:
+                _check_symmetric(A)
     be = _backend_or_default(backend)
     for wide in (False, True):
         Ns = sorted(N for N in groups if (N > DP_MAX) == wide)
         if not Ns:
             continue
-        Ps = [_gather_pairs_batch(T, np.stack([As[i] for i in groups[N]]), None, N) for N in Ns]
+        Ps = [_gather_pairs_batch(T, stacks[N], None, N) for N in Ns]
         res = _haf_dp_backend_mixed(be, Ps, [_plan_for(N, backend) for N in Ns], max_batch=max_batch)
         for N, r in zip(Ns, res):
             out[groups[N]] = r
