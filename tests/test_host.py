@@ -323,3 +323,27 @@ def test_symmetry_check_semantics():
     Z2[0, 7] += 1j * 1e-3
     with pytest.raises(te.ArgumentError):
         _check_symmetric(Z2)
+
+
+def test_batch_refusals_before_the_device():
+    # the batch forms name the first bad pattern or matrix before any device call
+    A = np.ones((4, 4))
+    with pytest.raises(te.DimensionMismatch, match="rpt 2 has length 3"):
+        te.hafnian_repeated(A, [[1, 1, 0, 0], [1, 1, 0]])
+    with pytest.raises(te.ArgumentError, match="rpt 2 contains a negative entry"):
+        te.hafnian_repeated(A, [[1, 1, 0, 0], [3, -1, 0, 0]])
+    with pytest.raises(te.ArgumentError, match="rpt 3 sums to 4, but rpt 1 sums to 2"):
+        te.hafnian_repeated(A, [[1, 1, 0, 0], [0, 0, 1, 1], [1, 1, 1, 1]])
+    S = synth.sym_normal(6, 1, 0, 0.3)
+    bad = S.copy()
+    bad[2, 4] += 0.5
+    with pytest.raises(te.ArgumentError, match=r"not symmetric at \(3, 5\)"):
+        te.hafnian([S, S, bad])
+    with pytest.raises(te.DimensionMismatch, match="matrix 2 is 4×4, expected 6×6"):
+        te.hafnian([S, np.ones((4, 4))])
+    with pytest.raises(te.ArgumentError, match=r"not symmetric at \(3, 5\)"):
+        te.hafnian_mixed([np.ones((4, 4)), S, bad])
+    odd = np.zeros((5, 5))
+    odd[0, 1] = 1.0
+    with pytest.raises(te.ArgumentError, match=r"not symmetric at \(1, 2\)"):
+        te.hafnian_mixed([S, odd])
